@@ -1,0 +1,352 @@
+// capi.cpp — the extern "C" boundary declared in include/bsz_hip.h.
+#include "../../include/bsz_hip.h"
+
+#include "bsdata_io.h"
+#include "device.h"
+#include "executor.h"
+#include "fused.h"
+#include "frontend.h"
+#include "run_json.h"
+
+#include <chrono>
+#include <cstring>
+#include <new>
+
+using namespace bsz;
+
+struct bsz_ctx {
+    int device = 0;
+    hipStream_t s = nullptr;
+    std::string err;
+    DevPool pool;
+    std::unordered_map<uint64_t*, size_t> bufs;
+    std::unique_ptr<Engine> perop, fused;
+    void* add_state = nullptr;
+    size_t add_state_bytes = 0;
+    unsigned long long* slot = nullptr;
+};
+
+struct bsz_program {
+    std::unique_ptr<Program> prog;
+    std::string source;
+};
+
+struct bsz_case {
+    CaseData c;
+};
+
+struct bsz_result {
+    RunOutcome out;
+};
+
+namespace {
+
+thread_local std::string t_err;
+
+int fail(bsz_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    t_err = msg;
+    return code;
+}
+
+template <class F>
+int guard(bsz_ctx* ctx, F&& f) {
+    try {
+        if (ctx) BSZ_HIP(hipSetDevice(ctx->device));
+        f();
+        return BSZ_OK;
+    } catch (const ParseError& e) {
+        return fail(ctx, BSZ_E_PARSE, e.what());
+    } catch (const RunError& e) {
+        return fail(ctx, BSZ_E_RUNTIME, e.what());
+    } catch (const HipError& e) {
+        std::string m = e.what();
+        return fail(ctx, m.find("out of memory") != std::string::npos ? BSZ_E_OOM : BSZ_E_HIP, m);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, BSZ_E_OOM, "host out of memory");
+    } catch (const std::exception& e) {
+        return fail(ctx, BSZ_E_RUNTIME, e.what());
+    }
+}
+
+Geom geom_of(int64_t bitlength) { return Geom::make(bitlength, 1); }
+
+void ensure_scalar_state(bsz_ctx* ctx, const Geom& g) {
+    size_t need = AddScanState::bytes(g);
+    if (need > ctx->add_state_bytes) {
+        if (ctx->add_state) { BSZ_HIP(hipStreamSynchronize(ctx->s)); BSZ_HIP(hipFree(ctx->add_state)); }
+        BSZ_HIP(hipMalloc(&ctx->add_state, need));
+        BSZ_HIP(hipMemsetAsync(ctx->add_state, 0, need, ctx->s));
+        ctx->add_state_bytes = need;
+    }
+    if (!ctx->slot) BSZ_HIP(hipMalloc(&ctx->slot, 64));
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsz_ctx_create(bsz_ctx** out, int device) {
+    if (!out) return fail(nullptr, BSZ_E_INVALID, "null out");
+    auto* c = new (std::nothrow) bsz_ctx;
+    if (!c) return fail(nullptr, BSZ_E_OOM, "host out of memory");
+    c->device = device;
+    int rc = guard(c, [&] { BSZ_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking)); });
+    if (rc != BSZ_OK) { t_err = c->err; delete c; return rc; }
+    *out = c;
+    return BSZ_OK;
+}
+
+int bsz_ctx_destroy(bsz_ctx* ctx) {
+    if (!ctx) return BSZ_OK;
+    int rc = guard(ctx, [&] {
+        BSZ_HIP(hipStreamSynchronize(ctx->s));
+        ctx->perop.reset();
+        ctx->fused.reset();
+        for (auto& kv : ctx->bufs) ctx->pool.put(kv.first, kv.second);
+        ctx->bufs.clear();
+        ctx->pool.trim();
+        if (ctx->add_state) (void)hipFree(ctx->add_state);
+        if (ctx->slot) (void)hipFree(ctx->slot);
+        (void)hipStreamDestroy(ctx->s);
+    });
+    delete ctx;
+    return rc;
+}
+
+const char* bsz_last_error(const bsz_ctx* ctx) { return ctx ? ctx->err.c_str() : t_err.c_str(); }
+
+int bsz_sync(bsz_ctx* ctx) { return guard(ctx, [&] { BSZ_HIP(hipStreamSynchronize(ctx->s)); }); }
+
+void* bsz_ctx_stream(bsz_ctx* ctx) { return ctx ? (void*)ctx->s : nullptr; }
+
+uint64_t bsz_stream_alloc_words(int64_t bitlength) { return geom_of(bitlength).alloc_words(); }
+
+int bsz_buf_alloc(bsz_ctx* ctx, int64_t bitlength, uint64_t** dev_out) {
+    return guard(ctx, [&] {
+        Geom g = geom_of(bitlength);
+        auto* p = (uint64_t*)ctx->pool.get(g.bytes());
+        BSZ_HIP(hipMemsetAsync(p, 0, g.bytes(), ctx->s));
+        ctx->bufs[p] = g.bytes();
+        *dev_out = p;
+    });
+}
+
+int bsz_buf_free(bsz_ctx* ctx, uint64_t* dev) {
+    return guard(ctx, [&] {
+        auto it = ctx->bufs.find(dev);
+        if (it == ctx->bufs.end()) throw std::invalid_argument("bsz_buf_free: unknown buffer");
+        BSZ_HIP(hipStreamSynchronize(ctx->s));
+        ctx->pool.put(it->first, it->second);
+        ctx->bufs.erase(it);
+    });
+}
+
+int bsz_h2d(bsz_ctx* ctx, uint64_t* dev, const uint64_t* host, uint64_t n, int64_t bitlength) {
+    return guard(ctx, [&] {
+        upload_stream(ctx->s, dev, host, n, geom_of(bitlength));
+        BSZ_HIP(hipStreamSynchronize(ctx->s));
+    });
+}
+
+int bsz_d2h(bsz_ctx* ctx, uint64_t* host, const uint64_t* dev, uint64_t n) {
+    return guard(ctx, [&] {
+        BSZ_HIP(hipMemcpyAsync(host, dev, n * 8, hipMemcpyDeviceToHost, ctx->s));
+        BSZ_HIP(hipStreamSynchronize(ctx->s));
+    });
+}
+
+int bsz_op_fill(bsz_ctx* ctx, uint64_t* dst, int ones, int64_t bl) {
+    return guard(ctx, [&] { launch_fill(ctx->s, dst, ones != 0, geom_of(bl)); });
+}
+int bsz_op_copy(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, int64_t bl) {
+    return guard(ctx, [&] { launch_pointwise(ctx->s, PwOp::Copy, dst, src, nullptr, geom_of(bl)); });
+}
+int bsz_op_not(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, int64_t bl) {
+    return guard(ctx, [&] { launch_pointwise(ctx->s, PwOp::Not, dst, a, nullptr, geom_of(bl)); });
+}
+int bsz_op_and(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bl) {
+    return guard(ctx, [&] { launch_pointwise(ctx->s, PwOp::And, dst, a, b, geom_of(bl)); });
+}
+int bsz_op_or(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bl) {
+    return guard(ctx, [&] { launch_pointwise(ctx->s, PwOp::Or, dst, a, b, geom_of(bl)); });
+}
+int bsz_op_xor(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bl) {
+    return guard(ctx, [&] { launch_pointwise(ctx->s, PwOp::Xor, dst, a, b, geom_of(bl)); });
+}
+int bsz_op_add(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bl) {
+    return guard(ctx, [&] {
+        if (dst == a || dst == b) throw std::invalid_argument("bsz_op_add: dst aliases an operand");
+        Geom g = geom_of(bl);
+        ensure_scalar_state(ctx, g);
+        launch_add(ctx->s, dst, a, b, g, ctx->add_state);
+    });
+}
+int bsz_op_shl(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, uint64_t k, int64_t bl) {
+    return guard(ctx, [&] {
+        if (dst == src) throw std::invalid_argument("bsz_op_shl: dst aliases src");
+        launch_shl(ctx->s, dst, src, k, geom_of(bl));
+    });
+}
+int bsz_op_shr(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, uint64_t k, int64_t bl) {
+    return guard(ctx, [&] {
+        if (dst == src) throw std::invalid_argument("bsz_op_shr: dst aliases src");
+        launch_shr(ctx->s, dst, src, k, geom_of(bl));
+    });
+}
+int bsz_op_popcount(bsz_ctx* ctx, const uint64_t* src, int64_t bl, int64_t* out) {
+    return guard(ctx, [&] {
+        Geom g = geom_of(bl);
+        ensure_scalar_state(ctx, g);
+        launch_zero_u64(ctx->s, ctx->slot, 1);
+        launch_popcount(ctx->s, ctx->slot, src, g);
+        unsigned long long v = 0;
+        BSZ_HIP(hipMemcpyAsync(&v, ctx->slot, 8, hipMemcpyDeviceToHost, ctx->s));
+        BSZ_HIP(hipStreamSynchronize(ctx->s));
+        *out = (int64_t)v;
+    });
+}
+int bsz_op_is_nonzero(bsz_ctx* ctx, const uint64_t* src, int64_t bl, int* out) {
+    return guard(ctx, [&] {
+        Geom g = geom_of(bl);
+        ensure_scalar_state(ctx, g);
+        launch_zero_u64(ctx->s, ctx->slot, 1);
+        launch_nonzero(ctx->s, ctx->slot, src, g);
+        unsigned long long v = 0;
+        BSZ_HIP(hipMemcpyAsync(&v, ctx->slot, 8, hipMemcpyDeviceToHost, ctx->s));
+        BSZ_HIP(hipStreamSynchronize(ctx->s));
+        *out = v ? 1 : 0;
+    });
+}
+
+int bsz_program_parse(const char* source, bsz_program** out) {
+    return guard(nullptr, [&] {
+        auto p = std::make_unique<bsz_program>();
+        p->source = source;
+        p->prog = parse_program(p->source);
+        *out = p.release();
+    });
+}
+
+int bsz_program_free(bsz_program* p) { delete p; return BSZ_OK; }
+
+int64_t bsz_program_static_ops(const bsz_program* p, const bsz_case* c) {
+    return count_static_ops(*p->prog, c->c.params);
+}
+
+int bsz_case_create(bsz_case** out, int64_t bitlength) {
+    auto* c = new (std::nothrow) bsz_case;
+    if (!c) return fail(nullptr, BSZ_E_OOM, "host out of memory");
+    c->c.bitlength = bitlength;
+    *out = c;
+    return BSZ_OK;
+}
+int bsz_case_free(bsz_case* c) { delete c; return BSZ_OK; }
+int bsz_case_set_param(bsz_case* c, const char* n, int64_t v) { c->c.params[n] = v; return BSZ_OK; }
+int bsz_case_set_input(bsz_case* c, const char* n, const uint64_t* w, uint64_t k) {
+    c->c.inputs[n].assign(w, w + k);
+    return BSZ_OK;
+}
+int bsz_case_set_array_elem(bsz_case* c, const char* n, int64_t idx, const uint64_t* w, uint64_t k) {
+    c->c.input_arrays[n][idx].assign(w, w + k);
+    return BSZ_OK;
+}
+int bsz_case_set_broadcast(bsz_case* c, const char* n, const uint8_t* bits, int64_t count) {
+    std::vector<bool> v((size_t)std::max<int64_t>(0, count));
+    for (int64_t i = 0; i < count; ++i) v[i] = (bits[i / 8] >> (i % 8)) & 1;
+    c->c.broadcast[n] = std::move(v);
+    return BSZ_OK;
+}
+int bsz_case_set_iterations(bsz_case* c, int it) { c->c.iterations = it; return BSZ_OK; }
+int bsz_case_set_feedback(bsz_case* c, const char* o, const char* i) { c->c.feedback[o] = i; return BSZ_OK; }
+
+int bsz_case_read_bsdata(const char* path, bsz_case*** cases_out, int* n_out) {
+    try {
+        auto v = read_bsdata_file(path);
+        auto** arr = new bsz_case*[v.size() ? v.size() : 1];
+        for (size_t i = 0; i < v.size(); ++i) {
+            arr[i] = new bsz_case;
+            arr[i]->c = std::move(v[i]);
+        }
+        *cases_out = arr;
+        *n_out = (int)v.size();
+        return BSZ_OK;
+    } catch (const std::exception& e) {
+        return fail(nullptr, BSZ_E_IO, e.what());
+    }
+}
+
+int bsz_case_array_free(bsz_case** cases, int n) {
+    for (int i = 0; i < n; ++i) delete cases[i];
+    delete[] cases;
+    return BSZ_OK;
+}
+
+int bsz_run(bsz_ctx* ctx, const bsz_program* p, const bsz_case* c, int executor, bsz_result** out) {
+    return guard(ctx, [&] {
+        Engine* eng = nullptr;
+        if (executor == BSZ_EXEC_FUSED) {
+            if (!ctx->fused) ctx->fused = make_fused_engine(ctx->s, FusedOptions::from_env());
+            eng = ctx->fused.get();
+        } else {
+            if (!ctx->perop) ctx->perop = make_perop_engine(ctx->s);
+            eng = ctx->perop.get();
+        }
+        auto r = std::make_unique<bsz_result>();
+        r->out = run_case_iterations(*p->prog, c->c, *eng, 0.0);
+        *out = r.release();
+    });
+}
+
+int bsz_result_free(bsz_result* r) { delete r; return BSZ_OK; }
+
+int bsz_result_stream(const bsz_result* r, const char* name, uint64_t* out, uint64_t n) {
+    auto it = r->out.res.streams.find(name);
+    if (it == r->out.res.streams.end()) return 0;
+    for (uint64_t i = 0; i < n; ++i) out[i] = i < it->second.size() ? it->second[i] : 0;
+    return 1;
+}
+
+int64_t bsz_result_array_count(const bsz_result* r, const char* name) {
+    auto it = r->out.res.arrays.find(name);
+    return it == r->out.res.arrays.end() ? 0 : (int64_t)it->second.size();
+}
+
+int bsz_result_array_indices(const bsz_result* r, const char* name, int64_t* idx, int64_t cap) {
+    auto it = r->out.res.arrays.find(name);
+    if (it == r->out.res.arrays.end()) return 0;
+    int64_t k = 0;
+    for (auto& e : it->second) { if (k < cap) idx[k] = e.first; ++k; }
+    return (int)k;
+}
+
+int bsz_result_array_elem(const bsz_result* r, const char* name, int64_t idx, uint64_t* out, uint64_t n) {
+    auto it = r->out.res.arrays.find(name);
+    if (it == r->out.res.arrays.end()) return 0;
+    auto e = it->second.find(idx);
+    if (e == it->second.end()) return 0;
+    for (uint64_t i = 0; i < n; ++i) out[i] = i < e->second.size() ? e->second[i] : 0;
+    return 1;
+}
+
+int bsz_result_int(const bsz_result* r, const char* name, int64_t* out) {
+    auto it = r->out.res.ints.find(name);
+    if (it == r->out.res.ints.end()) return 0;
+    *out = it->second;
+    return 1;
+}
+
+int64_t bsz_result_op_count(const bsz_result* r) { return r->out.dynamic_ops; }
+uint64_t bsz_result_n_words(const bsz_result* r) { return r->out.n_words; }
+
+int64_t bsz_result_json(const bsz_result* r, char* buf, uint64_t cap) {
+    std::string j = result_json(r->out);
+    if (buf && cap) {
+        size_t k = std::min<size_t>(j.size(), cap - 1);
+        std::memcpy(buf, j.data(), k);
+        buf[k] = 0;
+    }
+    return (int64_t)j.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// device.h — stream geometry, device buffers and the one-kernel-per-op HIP
+// data plane (the counterpart of CudaBackend::op_*, cuda_backend.cuh:53-153,
+// and of kernels.cuh / add_carry.cuh, written for gfx950).
+//
+// Layout in HBM: a stream of bitlength N occupies lw = ceil(N/64) 64-bit
+// words, bit 0 = LSB of word 0 (bsdata.py:25-34).  Buffers are allocated in
+// whole rows of 64 words (one wavefront-wide coalesced access of 512 B); every
+// word at or above lw and every bit at or above N is kept zero, so any kernel
+// may read a whole row without bounds checks and `>>` never sees stale bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace bsz {
+
+struct HipError : std::runtime_error { using std::runtime_error::runtime_error; };
+
+#define BSZ_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw ::bsz::HipError(std::string(#call) + ": " + hipGetErrorString(e_) +  \
+                                  (e_ == hipErrorOutOfMemory ? " (out of memory)" : "")); \
+    } while (0)
+
+constexpr uint64_t kRowWords = 64;
+
+struct Geom {
+    int64_t bitlength = 0;   // logical length in bits (0: unbounded, as bsim)
+    uint64_t lw = 1;         // logical words
+    uint64_t rows = 1;       // 64-word rows
+    uint64_t lastmask = ~0ull;  // valid bits of word lw-1
+
+    static Geom make(int64_t bitlength, uint64_t words_if_unbounded) {
+        Geom g;
+        g.bitlength = bitlength;
+        g.lw = bitlength > 0 ? (uint64_t)(bitlength + 63) / 64 : std::max<uint64_t>(1, words_if_unbounded);
+        g.rows = (g.lw + kRowWords - 1) / kRowWords;
+        uint64_t rem = bitlength > 0 ? (uint64_t)bitlength % 64 : 0;
+        g.lastmask = rem ? ((1ull << rem) - 1) : ~0ull;
+        return g;
+    }
+    uint64_t alloc_words() const { return rows * kRowWords; }
+    uint64_t bytes() const { return alloc_words() * 8; }
+};
+
+// Caching device allocator: freed blocks are kept per size and handed back.
+class DevPool {
+public:
+    ~DevPool();
+    void* get(size_t bytes);
+    void put(void* p, size_t bytes);
+    void trim();
+    int64_t n_mallocs() const { return n_mallocs_; }
+    size_t live_bytes() const { return live_; }
+    size_t peak_bytes() const { return peak_; }
+
+private:
+    std::multimap<size_t, void*> free_;
+    int64_t n_mallocs_ = 0;
+    size_t live_ = 0, peak_ = 0;
+};
+
+// A device stream buffer (clean padding), returned to its pool when dropped.
+struct DevBuf {
+    DevPool* pool = nullptr;
+    uint64_t* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf(DevPool* p, size_t b) : pool(p), ptr((uint64_t*)p->get(b)), bytes(b) {}
+    ~DevBuf() { if (ptr) pool->put(ptr, bytes); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+using DevBufP = std::shared_ptr<DevBuf>;
+
+// ── per-op kernels (kernels.hip); all asynchronous on `s` ──
+enum class PwOp : int { And = 0, Or = 1, Xor = 2, Not = 3, Copy = 4 };
+
+void launch_pointwise(hipStream_t s, PwOp op, uint64_t* dst, const uint64_t* a, const uint64_t* b, const Geom& g);
+void launch_fill(hipStream_t s, uint64_t* dst, bool ones, const Geom& g);
+void launch_shl(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t k, const Geom& g);
+void launch_shr(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t k, const Geom& g);
+// Single-pass stream addition: wave ballot carry scan + decoupled lookback
+// across tiles.  `scan_state` holds AddScanState::bytes(g) zero-initialised
+// bytes; the kernel leaves it reusable for the next launch.
+struct AddScanState {
+    static constexpr int kTileWords = 1024;  // 4 waves x 4 rows x 64 words
+    static uint64_t tiles(const Geom& g) { return (g.alloc_words() + kTileWords - 1) / kTileWords; }
+    static size_t bytes(const Geom& g) { return 64 + tiles(g) * 4; }
+};
+void launch_add(hipStream_t s, uint64_t* dst, const uint64_t* a, const uint64_t* b, const Geom& g, void* scan_state);
+// popcount / is_nonzero bounded to the bitlength, accumulated into *slot
+// (atomic add / atomic or); the caller zeroes the slot first.
+void launch_popcount(hipStream_t s, unsigned long long* slot, const uint64_t* src, const Geom& g);
+void launch_nonzero(hipStream_t s, unsigned long long* slot, const uint64_t* src, const Geom& g);
+void launch_zero_u64(hipStream_t s, unsigned long long* p, uint64_t n);
+// Load `n` host-side words into a row-padded buffer, truncated to the stream
+// (bytes_to_words, word_utils.h:91-108) and masked to the bitlength.
+void upload_stream(hipStream_t s, uint64_t* dst, const uint64_t* host, uint64_t n, const Geom& g);
+// Copy-engine bandwidth probe kernel: dst[i] = src[i] over n words.
+void launch_copy_words(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t n);
+
+int64_t kernel_launch_count();
+
+}  // namespace bsz

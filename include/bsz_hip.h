@@ -1,0 +1,129 @@
+/*
+ * bsz_hip.h — C ABI of the MI355X bitstream executor (libbsz_hip.so).
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point returns
+ * BSZ_OK (0) or a negative bsz_status; bsz_last_error() gives the message.
+ * A context owns one HIP stream on one device; calls on a context are
+ * asynchronous on that stream unless they return host data.  One context per
+ * host thread.  Host pointers are borrowed for the duration of the call.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference tree).
+ */
+#ifndef BSZ_HIP_H
+#define BSZ_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    BSZ_OK = 0,
+    BSZ_E_INVALID = -1,   /* bad argument / unknown name */
+    BSZ_E_PARSE = -2,     /* .bs lexer, parser or 3-address error (main.cu:301-306) */
+    BSZ_E_RUNTIME = -3,   /* bs::RuntimeError (interpreter.h:22-25), e.g. while limit */
+    BSZ_E_HIP = -4,       /* HIP runtime error */
+    BSZ_E_OOM = -5,       /* device allocation failed ("out of memory", base.py:219-223) */
+    BSZ_E_IO = -6         /* .bsdata / file error (main.cu:267-270) */
+} bsz_status;
+
+typedef struct bsz_ctx bsz_ctx;
+typedef struct bsz_program bsz_program;
+typedef struct bsz_case bsz_case;
+typedef struct bsz_result bsz_result;
+typedef struct bsz_plan bsz_plan;
+
+/* ── context ─────────────────────────────────────────────────────────── */
+
+/* Replaces the per-case CudaBackend construction (main.cu:84-86). */
+int bsz_ctx_create(bsz_ctx** out, int device);
+int bsz_ctx_destroy(bsz_ctx* ctx);
+/* Last error message of ctx (or of the calling thread when ctx is NULL). */
+const char* bsz_last_error(const bsz_ctx* ctx);
+int bsz_sync(bsz_ctx* ctx);
+/* The context's hipStream_t, as an opaque pointer. */
+void* bsz_ctx_stream(bsz_ctx* ctx);
+
+/* ── device streams and one-kernel-per-op (the Backend concept,
+ *    cuda_backend.cuh:16-229 / cpu_backend.h:13-151) ─────────────────── */
+
+/* Words a stream of `bitlength` bits occupies on the device: ceil(bitlength/64)
+ * rounded up to a whole 64-word row.  Replaces GpuStream::alloc sizing
+ * (common.cuh:75-80, cuda_backend.cuh:23-25). */
+uint64_t bsz_stream_alloc_words(int64_t bitlength);
+/* alloc_stream (cuda_backend.cuh:39-44): zero-filled. */
+int bsz_buf_alloc(bsz_ctx* ctx, int64_t bitlength, uint64_t** dev_out);
+int bsz_buf_free(bsz_ctx* ctx, uint64_t* dev);
+/* load_from_words (cuda_backend.cuh:156-172): n host words, truncated or
+ * zero-padded to the stream and masked to the bitlength. */
+int bsz_h2d(bsz_ctx* ctx, uint64_t* dev, const uint64_t* host, uint64_t n, int64_t bitlength);
+/* store_to_words (cuda_backend.cuh:174-189): first n words to host (sync). */
+int bsz_d2h(bsz_ctx* ctx, uint64_t* host, const uint64_t* dev, uint64_t n);
+/* zero_stream / ones_stream (cuda_backend.cuh:46-51). */
+int bsz_op_fill(bsz_ctx* ctx, uint64_t* dst, int ones, int64_t bitlength);
+/* copy_stream (cuda_backend.cuh:45). */
+int bsz_op_copy(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, int64_t bitlength);
+/* op_not / op_and / op_or / op_xor (cuda_backend.cuh:53-68). */
+int bsz_op_not(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, int64_t bitlength);
+int bsz_op_and(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bitlength);
+int bsz_op_or(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bitlength);
+int bsz_op_xor(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bitlength);
+/* op_add (cuda_backend.cuh:69-74 + add_carry.cuh:110-135): one kernel,
+ * wave ballot carries + decoupled lookback.  dst must not alias a or b. */
+int bsz_op_add(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bitlength);
+/* op_shl / op_shr (cuda_backend.cuh:75-84).  dst must not alias src. */
+int bsz_op_shl(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, uint64_t k, int64_t bitlength);
+int bsz_op_shr(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, uint64_t k, int64_t bitlength);
+/* op_popcount / op_is_nonzero (cuda_backend.cuh:87-153), bounded to the
+ * bitlength; synchronous, results to host. */
+int bsz_op_popcount(bsz_ctx* ctx, const uint64_t* src, int64_t bitlength, int64_t* out);
+int bsz_op_is_nonzero(bsz_ctx* ctx, const uint64_t* src, int64_t bitlength, int* out);
+
+/* ── programs and cases (bs::parse, parser.cpp:451-457; BsdataInput,
+ *    bsdata_reader.h:31-40) ─────────────────────────────────────────── */
+
+int bsz_program_parse(const char* source, bsz_program** out);
+int bsz_program_free(bsz_program* p);
+/* count_ops_ast (ast_liveness.cpp:723-727) for the given case's params. */
+int64_t bsz_program_static_ops(const bsz_program* p, const bsz_case* c);
+
+int bsz_case_create(bsz_case** out, int64_t bitlength);
+int bsz_case_free(bsz_case* c);
+int bsz_case_set_param(bsz_case* c, const char* name, int64_t value);
+int bsz_case_set_input(bsz_case* c, const char* name, const uint64_t* words, uint64_t n);
+int bsz_case_set_array_elem(bsz_case* c, const char* name, int64_t idx, const uint64_t* words, uint64_t n);
+/* count elements, one bit each (1 = ONES), LSB first (bsdata.py:96-102). */
+int bsz_case_set_broadcast(bsz_case* c, const char* name, const uint8_t* bits, int64_t count);
+int bsz_case_set_iterations(bsz_case* c, int iterations);
+int bsz_case_set_feedback(bsz_case* c, const char* out_name, const char* in_name);
+/* All cases of a .bsdata file (read_bsdata_multi, bsdata_reader.h:262-299). */
+int bsz_case_read_bsdata(const char* path, bsz_case*** cases_out, int* n_out);
+int bsz_case_array_free(bsz_case** cases, int n);
+
+/* ── execution (Interpreter<Backend>::run, interpreter.inl:234-364, with
+ *    the iterations/feedback loop of main.cu:84-140) ─────────────────── */
+
+typedef enum { BSZ_EXEC_PEROP = 0, BSZ_EXEC_FUSED = 1 } bsz_executor;
+
+int bsz_run(bsz_ctx* ctx, const bsz_program* p, const bsz_case* c, int executor, bsz_result** out);
+int bsz_result_free(bsz_result* r);
+/* Stream output: returns 1 and writes min(n, lw) words masked to the
+ * bitlength, 0 when the program never produced it. */
+int bsz_result_stream(const bsz_result* r, const char* name, uint64_t* out, uint64_t n);
+int64_t bsz_result_array_count(const bsz_result* r, const char* name);
+int bsz_result_array_indices(const bsz_result* r, const char* name, int64_t* idx, int64_t cap);
+int bsz_result_array_elem(const bsz_result* r, const char* name, int64_t idx, uint64_t* out, uint64_t n);
+int bsz_result_int(const bsz_result* r, const char* name, int64_t* out);
+int64_t bsz_result_op_count(const bsz_result* r);
+uint64_t bsz_result_n_words(const bsz_result* r);
+/* The per-case JSON object bsim_cuda prints (main.cu:146-205); writes at most
+ * cap bytes (NUL-terminated) and returns the full length. */
+int64_t bsz_result_json(const bsz_result* r, char* buf, uint64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
