@@ -1,0 +1,329 @@
+"""Benchmark of the fused MI355X bitstream executor.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload adder32|matchstar|shift_and]
+                    [--no-cpu] [--no-perop] [--no-extra]
+
+A step is one pass of the program over one batch of synthetic input already
+resident in HBM.  The headline workload is the bitsliced 32-bit ripple-carry
+adder DAG (65 input streams of 2^28 bits, 193 DSL ops, 33 outputs and 33
+popcounts); MatchStar on 2^26 bits and the batched Shift-And of 2048 uneven
+sequences are measured beside it.  Each is timed with hipEvents on the
+executor's stream over K replays of a captured hipGraph, next to the plain
+one-kernel-per-op executor on the same resident buffers, and its outputs are
+checked bit-for-bit against an independent host computation at full size.
+The CPU baseline is the AVX-512 + OpenMP CPU executor (oracle/) on a bounded
+sample of the same workload.  Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from bitstreamzoo_amd import api, workloads  # noqa: E402
+from bitstreamzoo_amd.bsdata import BsCase  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "stream Gbit-ops/s + % HBM roofline; learned vs autotuned schedule time"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def words_popcount(w: np.ndarray) -> int:
+    return int(np.unpackbits(w.view(np.uint8)).sum())
+
+
+def shape_case(w: workloads.Workload) -> api.Case:
+    c0 = w.cases[0]
+    small = BsCase(bitlength=64, params=dict(c0.params),
+                   inputs={k: np.zeros(1, np.uint64) for k in c0.inputs},
+                   input_arrays={k: {i: np.zeros(1, np.uint64) for i in a} for k, a in c0.input_arrays.items()})
+    return api.Case.from_bscase(small)
+
+
+class Bound:
+    """A plan with every instance's inputs uploaded and bound."""
+
+    def __init__(self, ctx, w, cases, options=None):
+        self.ctx = ctx
+        self.plan = api.Plan(ctx, api.Program(w.source), shape_case(w), options or {})
+        self.plan.set_instances([c.bitlength for c in cases])
+        self.bufs = []
+        for i, c in enumerate(cases):
+            for n, words in c.inputs.items():
+                d = ctx.upload(words, c.bitlength)
+                self.bufs.append(d)
+                self.plan.bind(i, n, d)
+        self.plan.prepare()
+
+    def free(self):
+        self.plan.close()
+        for d in self.bufs:
+            d.free()
+
+
+def time_graph(ctx, graph, steps, warmup, reps_per_launch=1):
+    graph.launch(warmup)
+    ctx.sync()
+    ctx.timer_start()
+    graph.launch(steps)
+    ms = ctx.timer_stop()
+    return ms / (steps * reps_per_launch)
+
+
+def time_plans(ctx, plans, steps, warmup, executor):
+    """Capture one graph that runs every plan once; time `steps` replays.
+    Returns ms per plan run."""
+    for p in plans:
+        p.set_executor(executor)
+        p.prepare()
+        p.run()
+    ctx.sync()
+    g = ctx.capture(lambda: [p.run() for p in plans])
+    ms = time_graph(ctx, g, steps, warmup, reps_per_launch=len(plans))
+    del g
+    return ms
+
+
+# ── full-size host checks (independent of the executors) ──
+
+def adder_expected(case: BsCase, width: int):
+    a = [case.inputs[f"a{i}"] for i in range(width)]
+    b = [case.inputs[f"b{i}"] for i in range(width)]
+    c = case.inputs["cin"]
+    outs = {}
+    for i in range(width):
+        p = a[i] ^ b[i]
+        outs[f"s{i}"] = p ^ c
+        c = (a[i] & b[i]) | (p & c)
+    outs["cout"] = c
+    return outs
+
+
+def matchstar_expected(case: BsCase):
+    bl = case.bitlength
+    m = (1 << bl) - 1
+    M = int.from_bytes(case.inputs["M"].tobytes(), "little")
+    Cc = int.from_bytes(case.inputs["C"].tobytes(), "little")
+    r = ((((M & Cc) + Cc) & m) ^ Cc) | M
+    return np.frombuffer(r.to_bytes(len(case.inputs["M"]) * 8, "little"), dtype=np.uint64)
+
+
+def shift_and_expected_popcount(case: BsCase, pattern: str) -> int:
+    bl = case.bitlength
+    m = (1 << bl) - 1
+    acc = m
+    for i, ch in enumerate(pattern):
+        v = int.from_bytes(case.inputs[f"c{ch}"].tobytes(), "little")
+        acc &= (v << (len(pattern) - 1 - i)) & m
+    return bin(acc).count("1")
+
+
+# ── workloads ──
+
+def bench_adder(ctx, args, perop=True):
+    bl = args.bitlength or (1 << 28)
+    log(f"adder32: generating 65 x {bl} bits")
+    w = workloads.adder32(bl, seed=2)
+    case = w.cases[0]
+    B = Bound(ctx, w, [case])
+    B.plan.run()
+    ctx.sync()
+    log("adder32: checking all 33 outputs and popcounts against the host computation")
+    exp = adder_expected(case, 32)
+    nw = len(case.inputs["a0"])
+    ok = True
+    for name, e in exp.items():
+        got = B.plan.download_output(0, name, nw)
+        if not np.array_equal(got, e):
+            ok = False
+            log(f"MISMATCH in output {name}")
+        if B.plan.read_int(0, "n_" + name) != words_popcount(e):
+            ok = False
+            log(f"MISMATCH in popcount n_{name}")
+    if not ok:
+        raise SystemExit("adder32 outputs differ from the host computation")
+    ms = time_plans(ctx, [B.plan], args.steps, args.warmup, api.FUSED)
+    res = {"ms_per_step": ms, "bit_ops": w.bit_ops, "alg_bytes": w.alg_bytes, "plan": B.plan.info()}
+    if perop:
+        res["perop_ms_per_step"] = time_plans(ctx, [B.plan], max(2, args.steps // 4), 1, api.PEROP)
+    B.free()
+    return w, res
+
+
+def bench_matchstar(ctx, args):
+    bl = 1 << 26
+    rot = 12  # distinct input sets, so the 256 MB last-level cache cannot serve them
+    log(f"matchstar: generating {rot} x 2 x {bl} bits")
+    ws = [workloads.matchstar(bl, seed=100 + r) for r in range(rot)]
+    bound = [Bound(ctx, w, w.cases) for w in ws]
+    for B, w in zip(bound, ws):
+        B.plan.run()
+    ctx.sync()
+    for B, w in zip(bound, ws):
+        e = matchstar_expected(w.cases[0])
+        got = B.plan.download_output(0, "result", len(e))
+        if not np.array_equal(got, e) or B.plan.read_int(0, "n") != words_popcount(e):
+            raise SystemExit("matchstar output differs from the host computation")
+    ms = time_plans(ctx, [b.plan for b in bound], args.steps, args.warmup, api.FUSED)
+    perop = time_plans(ctx, [b.plan for b in bound], max(2, args.steps // 4), 1, api.PEROP)
+    info = bound[0].plan.info()
+    for B in bound:
+        B.free()
+    return ws[0], {"ms_per_step": ms, "perop_ms_per_step": perop, "bit_ops": ws[0].bit_ops,
+                   "alg_bytes": ws[0].alg_bytes, "plan": info, "input_sets": rot}
+
+
+def bench_shift_and(ctx, args):
+    n = 2048
+    log(f"shift_and: generating {n} sequences, lengths log-uniform in [2^12, 2^22]")
+    w = workloads.shift_and_batch(n_seqs=n, seed=3)
+    B = Bound(ctx, w, w.cases, {"chunk_rows": 16, "rows_per_step": 4})
+    B.plan.run()
+    ctx.sync()
+    pattern = w.source.split("the 32-mer ")[1].split(":")[0]
+    rng = np.random.Generator(np.random.PCG64(7))
+    check_idx = sorted(set(rng.integers(0, n, size=16).tolist()) | {0, n - 1})
+    for i in check_idx:
+        if B.plan.read_int(i, "n") != shift_and_expected_popcount(w.cases[i], pattern):
+            raise SystemExit(f"shift_and popcount of sequence {i} differs from the host computation")
+    ms = time_plans(ctx, [B.plan], args.steps, args.warmup, api.FUSED)
+    perop = time_plans(ctx, [B.plan], 2, 1, api.PEROP)
+    info = B.plan.info()
+    B.free()
+    return w, {"ms_per_step": ms, "perop_ms_per_step": perop, "bit_ops": w.bit_ops, "alg_bytes": w.alg_bytes,
+               "plan": info, "instances": n, "checked_instances": len(check_idx)}
+
+
+def copy_bandwidth(ctx) -> float:
+    """Achieved GB/s of a plain device copy (read + write) of 2 GiB."""
+    bl = 1 << 34  # 2 GiB per buffer
+    src = ctx.alloc(bl)
+    dst = ctx.alloc(bl)
+    g = ctx.capture(lambda: api.lib().bsz_op_copy(ctx.h, dst.dev, src.dev, bl))
+    ms = time_graph(ctx, g, 10, 2)
+    src.free()
+    dst.free()
+    return 2 * (bl // 8) / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(sample_bits: int):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from cpu_models import Oracle
+    threads = min(16, os.cpu_count() or 1)
+    Oracle.lib().bszo_set_threads(threads)
+    threads = Oracle.lib().bszo_get_threads()
+    w = workloads.adder32(sample_bits, seed=5)
+    o = Oracle(w.source).load(w.cases[0])
+    o.run(Oracle.FAITHFUL)  # warm
+    t0 = time.perf_counter()
+    o = Oracle(w.source).load(w.cases[0])
+    o.run(Oracle.FAITHFUL)
+    wall = time.perf_counter() - t0
+    ms = o.exec_ms()
+    return {"value": w.bit_ops / (ms * 1e-3) / 1e9, "unit": "Gbit-ops/s", "cores": threads, "kind": "port",
+            "sample": f"adder32 DAG over {sample_bits} bits (1/{(1 << 28) // sample_bits} of the workload), "
+                      f"one AVX-512+OpenMP pass per DSL op, {ms:.1f} ms exec ({wall:.1f} s wall)",
+            "avx512": bool(Oracle.lib().bszo_has_avx512())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="adder32", choices=["adder32", "matchstar", "shift_and"])
+    ap.add_argument("--bitlength", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-perop", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the executor runs on a single MI355X and this build has no "
+              "multi-GPU path (see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    ctx = api.Context(0)
+    results = {}
+    extra = {}
+    if args.workload == "adder32":
+        w, r = bench_adder(ctx, args, perop=not args.no_perop)
+    elif args.workload == "matchstar":
+        w, r = bench_matchstar(ctx, args)
+    else:
+        w, r = bench_shift_and(ctx, args)
+    results[args.workload] = r
+    if not args.no_extra and args.workload == "adder32":
+        for name, fn in (("matchstar", bench_matchstar), ("shift_and", bench_shift_and)):
+            try:
+                _, rr = fn(ctx, args)
+                extra[name] = rr
+            except SystemExit:
+                raise
+            except Exception as e:  # report, do not hide
+                extra[name] = {"error": str(e)}
+    copy_gbps = copy_bandwidth(ctx)
+
+    ms = r["ms_per_step"]
+    gbitops = r["bit_ops"] / (ms * 1e-3) / 1e9
+    achieved = r["alg_bytes"] / (ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(gbitops, 1),
+        "unit": "Gbit-ops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded PCG64), inputs resident in HBM",
+        "config": {"workload": {
+            "adder32": "bitsliced 32-bit ripple-carry adder DAG, 65 inputs x 2^28 bits, 160 ops + 33 popcounts",
+            "matchstar": "MatchStar on 2 x 2^26-bit streams + popcount, 12 rotating input sets",
+            "shift_and": "batched Shift-And, 2048 sequences, 32-mer, one launch"}[args.workload],
+            "executor": "fused", "bitlength": w.cases[0].bitlength if args.workload != "shift_and" else None,
+            "kernels": len(r["plan"]["kernels"]), "chunk_rows": r["plan"]["chunk_rows"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "measured_copy_GBps": round(copy_gbps, 1)},
+    }
+    if "perop_ms_per_step" in r:
+        line["perop"] = {"ms_per_step": round(r["perop_ms_per_step"], 5),
+                         "value": round(r["bit_ops"] / (r["perop_ms_per_step"] * 1e-3) / 1e9, 1),
+                         "speedup_fused_over_perop": round(r["perop_ms_per_step"] / ms, 2)}
+    ex = {}
+    for name, rr in extra.items():
+        if "error" in rr:
+            ex[name] = rr
+            continue
+        m = rr["ms_per_step"]
+        ex[name] = {"ms_per_step": round(m, 5), "value": round(rr["bit_ops"] / (m * 1e-3) / 1e9, 1),
+                    "GBps": round(rr["alg_bytes"] / (m * 1e-3) / 1e9, 1),
+                    "hbm_frac": round(rr["alg_bytes"] / (m * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                    "perop_ms_per_step": round(rr["perop_ms_per_step"], 5),
+                    "speedup_fused_over_perop": round(rr["perop_ms_per_step"] / m, 2)}
+    if ex:
+        line["other_configs"] = ex
+    if not args.no_cpu:
+        log("cpu baseline: oracle AVX-512+OpenMP on a 2^24-bit adder32 sample")
+        line["cpu_baseline"] = cpu_baseline(1 << 24)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -66,6 +66,27 @@ _SIGS = {
     "bsz_result_op_count": (C.c_int64, [vp]),
     "bsz_result_n_words": (C.c_uint64, [vp]),
     "bsz_result_json": (C.c_int64, [vp, C.c_char_p, C.c_uint64]),
+    "bsz_plan_create": (C.c_int, [vp, vp, vp, C.c_char_p, C.POINTER(vp)]),
+    "bsz_plan_compile_only": (C.c_int64, [vp, vp, C.c_char_p, C.c_char_p, C.c_uint64]),
+    "bsz_plan_free": (C.c_int, [vp]),
+    "bsz_plan_set_instances": (C.c_int, [vp, C.c_int, i64p]),
+    "bsz_plan_bind_input": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int64, u64p]),
+    "bsz_plan_output": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int64, C.POINTER(u64p)]),
+    "bsz_plan_run": (C.c_int, [vp]),
+    "bsz_plan_capture": (C.c_int, [vp]),
+    "bsz_plan_replay": (C.c_int, [vp, C.c_int]),
+    "bsz_plan_read_int": (C.c_int, [vp, C.c_int, C.c_char_p, i64p]),
+    "bsz_plan_mismatches": (C.c_int, [vp, C.c_int, i64p]),
+    "bsz_plan_info": (C.c_int64, [vp, C.c_char_p, C.c_uint64]),
+    "bsz_plan_source": (C.c_int64, [vp, C.c_char_p, C.c_uint64]),
+    "bsz_plan_set_executor": (C.c_int, [vp, C.c_int]),
+    "bsz_plan_prepare": (C.c_int, [vp]),
+    "bsz_timer_start": (C.c_int, [vp]),
+    "bsz_timer_stop": (C.c_int, [vp, C.POINTER(C.c_double)]),
+    "bsz_capture_begin": (C.c_int, [vp]),
+    "bsz_capture_end": (C.c_int, [vp, C.POINTER(vp)]),
+    "bsz_graph_launch": (C.c_int, [vp, vp, C.c_int]),
+    "bsz_graph_destroy": (C.c_int, [vp]),
 }
 
 _lib = None

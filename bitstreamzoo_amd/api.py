@@ -69,6 +69,24 @@ class Context:
     def stream_handle(self) -> int:
         return lib().bsz_ctx_stream(self.h) or 0
 
+    def timer_start(self):
+        check(lib().bsz_timer_start(self.h), self.h)
+
+    def timer_stop(self) -> float:
+        v = C.c_double()
+        check(lib().bsz_timer_stop(self.h, C.byref(v)), self.h)
+        return v.value
+
+    def capture(self, fn) -> "Graph":
+        """Capture the launches `fn()` enqueues on this context into a graph."""
+        check(lib().bsz_capture_begin(self.h), self.h)
+        try:
+            fn()
+        finally:
+            g = C.c_void_p()
+            check(lib().bsz_capture_end(self.h, C.byref(g)), self.h)
+        return Graph(self, g)
+
     # ── streams and per-op kernels ──
     def alloc(self, bitlength: int) -> DevStream:
         p = _lib.u64p()
@@ -117,6 +135,19 @@ class Context:
         r = C.c_void_p()
         check(lib().bsz_run(self.h, program.h, case.h, executor, C.byref(r)), self.h)
         return Result(r, program)
+
+
+class Graph:
+    def __init__(self, ctx: "Context", h):
+        self.ctx, self.h = ctx, h
+
+    def launch(self, times: int = 1):
+        check(lib().bsz_graph_launch(self.ctx.h, self.h, times), self.ctx.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().bsz_graph_destroy(self.h)
+            self.h = None
 
 
 class Program:
@@ -216,3 +247,90 @@ class Result:
         buf = C.create_string_buffer(n + 1)
         L.bsz_result_json(self.h, buf, n + 1)
         return json.loads(buf.value.decode())
+
+
+def _get_str(fn, h) -> str:
+    n = fn(h, None, 0)
+    buf = C.create_string_buffer(n + 1)
+    fn(h, buf, n + 1)
+    return buf.value.decode()
+
+
+def compile_plan_only(program: "Program", shape: "Case", options: dict | None = None) -> dict:
+    """Record + generate + hipRTC-compile a fused plan (no GPU needed)."""
+    opts = json.dumps(options or {}).encode()
+    L = lib()
+    n = L.bsz_plan_compile_only(program.h, shape.h, opts, None, 0)
+    if n < 0:
+        check(int(n))
+    buf = C.create_string_buffer(n + 1)
+    L.bsz_plan_compile_only(program.h, shape.h, opts, buf, n + 1)
+    return json.loads(buf.value.decode())
+
+
+class Plan:
+    """A fused plan over a batch of instances (bsz_plan_*)."""
+
+    def __init__(self, ctx: Context, program: Program, shape: Case, options: dict | None = None):
+        h = C.c_void_p()
+        check(lib().bsz_plan_create(ctx.h, program.h, shape.h, json.dumps(options or {}).encode(), C.byref(h)),
+              ctx.h)
+        self.h, self.ctx, self.program = h, ctx, program
+
+    def close(self):
+        if self.h:
+            lib().bsz_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.close()
+
+    def set_instances(self, bitlengths):
+        arr = (C.c_int64 * len(bitlengths))(*[int(b) for b in bitlengths])
+        check(lib().bsz_plan_set_instances(self.h, len(bitlengths), arr), self.ctx.h)
+
+    def bind(self, inst: int, name: str, dev: DevStream, idx: int = -1):
+        check(lib().bsz_plan_bind_input(self.h, inst, name.encode(), idx, dev.dev), self.ctx.h)
+
+    def output_ptr(self, inst: int, name: str, idx: int = -1):
+        p = _lib.u64p()
+        check(lib().bsz_plan_output(self.h, inst, name.encode(), idx, C.byref(p)), self.ctx.h)
+        return p
+
+    def download_output(self, inst: int, name: str, n_words: int, idx: int = -1) -> np.ndarray:
+        p = self.output_ptr(inst, name, idx)
+        out = np.zeros(n_words, dtype=np.uint64)
+        check(lib().bsz_d2h(self.ctx.h, ptr(out), p, n_words), self.ctx.h)
+        return out
+
+    def run(self):
+        check(lib().bsz_plan_run(self.h), self.ctx.h)
+
+    def set_executor(self, executor: int):
+        check(lib().bsz_plan_set_executor(self.h, executor), self.ctx.h)
+
+    def prepare(self):
+        check(lib().bsz_plan_prepare(self.h), self.ctx.h)
+
+    def capture(self):
+        check(lib().bsz_plan_capture(self.h), self.ctx.h)
+
+    def replay(self, times: int = 1):
+        check(lib().bsz_plan_replay(self.h, times), self.ctx.h)
+
+    def read_int(self, inst: int, name: str) -> int:
+        v = C.c_int64()
+        check(lib().bsz_plan_read_int(self.h, inst, name.encode(), C.byref(v)), self.ctx.h)
+        return v.value
+
+    def mismatches(self, kernel: int = 0) -> int:
+        v = C.c_int64()
+        check(lib().bsz_plan_mismatches(self.h, kernel, C.byref(v)), self.ctx.h)
+        return v.value
+
+    def info(self) -> dict:
+        return json.loads(_get_str(lib().bsz_plan_info, self.h))
+
+    def source(self) -> str:
+        return _get_str(lib().bsz_plan_source, self.h)

@@ -24,6 +24,7 @@ struct bsz_ctx {
     void* add_state = nullptr;
     size_t add_state_bytes = 0;
     unsigned long long* slot = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
 };
 
 struct bsz_program {
@@ -37,6 +38,11 @@ struct bsz_case {
 
 struct bsz_result {
     RunOutcome out;
+};
+
+struct bsz_plan {
+    bsz_ctx* ctx = nullptr;
+    std::unique_ptr<FusedPlan> plan;
 };
 
 namespace {
@@ -108,6 +114,7 @@ int bsz_ctx_destroy(bsz_ctx* ctx) {
         ctx->pool.trim();
         if (ctx->add_state) (void)hipFree(ctx->add_state);
         if (ctx->slot) (void)hipFree(ctx->slot);
+        if (ctx->t0) { (void)hipEventDestroy(ctx->t0); (void)hipEventDestroy(ctx->t1); }
         (void)hipStreamDestroy(ctx->s);
     });
     delete ctx;
@@ -347,6 +354,127 @@ int64_t bsz_result_json(const bsz_result* r, char* buf, uint64_t cap) {
         buf[k] = 0;
     }
     return (int64_t)j.size();
+}
+
+
+static int64_t put_str(const std::string& j, char* buf, uint64_t cap) {
+    if (buf && cap) {
+        size_t k = std::min<size_t>(j.size(), cap - 1);
+        std::memcpy(buf, j.data(), k);
+        buf[k] = 0;
+    }
+    return (int64_t)j.size();
+}
+
+int bsz_plan_create(bsz_ctx* ctx, const bsz_program* p, const bsz_case* shape, const char* options_json,
+                    bsz_plan** out) {
+    return guard(ctx, [&] {
+        auto h = std::make_unique<bsz_plan>();
+        h->ctx = ctx;
+        h->plan = FusedPlan::record(ctx->s, *p->prog, shape->c,
+                                    FusedOptions::from_json(options_json ? options_json : ""), true);
+        *out = h.release();
+    });
+}
+
+int64_t bsz_plan_compile_only(const bsz_program* p, const bsz_case* shape, const char* options_json, char* info,
+                              uint64_t cap) {
+    std::string j;
+    int rc = guard(nullptr, [&] {
+        auto fp = FusedPlan::record(nullptr, *p->prog, shape->c,
+                                    FusedOptions::from_json(options_json ? options_json : ""), true);
+        j = fp->info_json();
+    });
+    if (rc != BSZ_OK) return rc;
+    return put_str(j, info, cap);
+}
+
+int bsz_plan_free(bsz_plan* plan) {
+    if (!plan) return BSZ_OK;
+    int rc = guard(plan->ctx, [&] { plan->plan.reset(); });
+    delete plan;
+    return rc;
+}
+
+int bsz_plan_set_instances(bsz_plan* plan, int n, const int64_t* bitlengths) {
+    return guard(plan->ctx, [&] { plan->plan->set_instances(std::vector<int64_t>(bitlengths, bitlengths + n)); });
+}
+
+int bsz_plan_bind_input(bsz_plan* plan, int inst, const char* name, int64_t idx, const uint64_t* dev) {
+    return guard(plan->ctx, [&] { plan->plan->bind_input(inst, name, idx, dev); });
+}
+
+int bsz_plan_output(bsz_plan* plan, int inst, const char* name, int64_t idx, uint64_t** dev_out) {
+    return guard(plan->ctx, [&] { *dev_out = plan->plan->output(inst, name, idx); });
+}
+
+int bsz_plan_run(bsz_plan* plan) { return guard(plan->ctx, [&] { plan->plan->run(); }); }
+int bsz_plan_capture(bsz_plan* plan) { return guard(plan->ctx, [&] { plan->plan->capture(); }); }
+int bsz_plan_replay(bsz_plan* plan, int times) {
+    return guard(plan->ctx, [&] { for (int i = 0; i < times; ++i) plan->plan->replay(); });
+}
+
+int bsz_plan_read_int(bsz_plan* plan, int inst, const char* name, int64_t* out) {
+    return guard(plan->ctx, [&] { *out = plan->plan->read_int(inst, name, true); });
+}
+
+int bsz_plan_mismatches(bsz_plan* plan, int kernel, int64_t* out) {
+    return guard(plan->ctx, [&] { *out = plan->plan->mismatches(kernel); });
+}
+
+int bsz_plan_set_executor(bsz_plan* plan, int executor) {
+    return guard(plan->ctx, [&] { plan->plan->set_executor(executor == BSZ_EXEC_PEROP ? 0 : 1); });
+}
+
+int bsz_plan_prepare(bsz_plan* plan) { return guard(plan->ctx, [&] { plan->plan->prepare_now(); }); }
+
+int bsz_timer_start(bsz_ctx* ctx) {
+    return guard(ctx, [&] {
+        if (!ctx->t0) { BSZ_HIP(hipEventCreate(&ctx->t0)); BSZ_HIP(hipEventCreate(&ctx->t1)); }
+        BSZ_HIP(hipEventRecord(ctx->t0, ctx->s));
+    });
+}
+
+int bsz_timer_stop(bsz_ctx* ctx, double* ms) {
+    return guard(ctx, [&] {
+        if (!ctx->t0) throw std::invalid_argument("bsz_timer_stop without bsz_timer_start");
+        BSZ_HIP(hipEventRecord(ctx->t1, ctx->s));
+        BSZ_HIP(hipEventSynchronize(ctx->t1));
+        float f = 0;
+        BSZ_HIP(hipEventElapsedTime(&f, ctx->t0, ctx->t1));
+        *ms = f;
+    });
+}
+
+int bsz_capture_begin(bsz_ctx* ctx) {
+    return guard(ctx, [&] { BSZ_HIP(hipStreamBeginCapture(ctx->s, hipStreamCaptureModeThreadLocal)); });
+}
+
+int bsz_capture_end(bsz_ctx* ctx, void** graph_exec) {
+    return guard(ctx, [&] {
+        hipGraph_t g;
+        hipGraphExec_t ex;
+        BSZ_HIP(hipStreamEndCapture(ctx->s, &g));
+        BSZ_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        *graph_exec = ex;
+    });
+}
+
+int bsz_graph_launch(bsz_ctx* ctx, void* graph_exec, int times) {
+    return guard(ctx, [&] {
+        for (int i = 0; i < times; ++i) BSZ_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, ctx->s));
+    });
+}
+
+int bsz_graph_destroy(void* graph_exec) {
+    if (graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec);
+    return BSZ_OK;
+}
+
+int64_t bsz_plan_info(const bsz_plan* plan, char* buf, uint64_t cap) { return put_str(plan->plan->info_json(), buf, cap); }
+int64_t bsz_plan_source(const bsz_plan* plan, char* buf, uint64_t cap) {
+    return put_str(plan->plan->module_source(), buf, cap);
 }
 
 }  // extern "C"

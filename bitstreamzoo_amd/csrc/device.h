@@ -104,6 +104,10 @@ void launch_zero_u64(hipStream_t s, unsigned long long* p, uint64_t n);
 // Load `n` host-side words into a row-padded buffer, truncated to the stream
 // (bytes_to_words, word_utils.h:91-108) and masked to the bitlength.
 void upload_stream(hipStream_t s, uint64_t* dst, const uint64_t* host, uint64_t n, const Geom& g);
+// Before a fused plan launch: for each of n_ctl control blocks of 4 words
+// (ticket, done, epoch, mismatches) clear ticket/done/mismatches and advance
+// the epoch; clear n_pop popcount accumulators.
+void launch_plan_prep(hipStream_t s, unsigned* ctl, uint32_t n_ctl, unsigned long long* pop, uint64_t n_pop);
 // Copy-engine bandwidth probe kernel: dst[i] = src[i] over n words.
 void launch_copy_words(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t n);
 

@@ -25,6 +25,36 @@ public:
     }
 
     RunResult run() {
+        walk();
+        E.flush();
+        return collect();
+    }
+
+    RecordedOutputs record() {
+        walk();
+        RecordedOutputs r;
+        for (Sym o : P.outputs) {
+            const std::string& nm = P.syms.names[o];
+            if (vars[o]) r.streams.emplace_back(nm, vars[o]);
+            auto ai = arrays.find(o);
+            if (ai != arrays.end())
+                for (auto& e : ai->second) r.array_elems.emplace_back(nm, e.first, e.second);
+        }
+        for (Sym o : P.out_ints) {
+            IntSlot& is = ints[o];
+            if (!is.set) continue;
+            RecordedOutputs::IntOut io;
+            io.name = P.syms.names[o];
+            io.pop = is.pop;
+            io.value = is.v;
+            r.ints.push_back(std::move(io));
+        }
+        r.op_count = ops;
+        return r;
+    }
+
+private:
+    void walk() {
         uint64_t max_words = 1;
         for (auto& kv : C.inputs) max_words = std::max<uint64_t>(max_words, kv.second.size());
         g = Geom::make(C.bitlength, max_words);
@@ -38,8 +68,9 @@ public:
         }
         load_inputs();
         for (const Stmt& s : P.stmts) exec(s);
-        E.flush();
+    }
 
+    RunResult collect() {
         RunResult r;
         for (Sym o : P.outputs) {
             const std::string& nm = P.syms.names[o];
@@ -73,7 +104,6 @@ public:
         return r;
     }
 
-private:
     struct IntSlot {
         bool set = false;
         int64_t v = 0;
@@ -115,6 +145,7 @@ private:
         for (Sym s : P.inputs) {
             auto it = C.inputs.find(nm(s));
             if (it != C.inputs.end()) {
+                E.note_input(nm(s), -1);
                 vars[s] = E.input(it->second.data(), it->second.size());
                 continue;
             }
@@ -122,7 +153,10 @@ private:
             auto ai = C.input_arrays.find(nm(s));
             if (ai != C.input_arrays.end()) {
                 auto& arr = arrays[s];
-                for (auto& e : ai->second) arr[(int64_t)(int32_t)e.first] = E.input(e.second.data(), e.second.size());
+                for (auto& e : ai->second) {
+                    E.note_input(nm(s), (int64_t)(int32_t)e.first);
+                    arr[(int64_t)(int32_t)e.first] = E.input(e.second.data(), e.second.size());
+                }
                 continue;
             }
             vars[s] = zero_c;
@@ -431,6 +465,10 @@ private:
 
 RunResult execute_program(const Program& prog, const CaseData& c, Engine& eng) {
     return Walker(prog, c, eng).run();
+}
+
+RecordedOutputs record_program(const Program& prog, const CaseData& c, Engine& eng) {
+    return Walker(prog, c, eng).record();
 }
 
 std::unique_ptr<Engine> make_perop_engine(hipStream_t s) { return std::make_unique<PerOpEngine>(s); }

@@ -38,6 +38,8 @@ struct StreamVal {
     DevBufP buf;
     int32_t node = -1;      // node id in the engine's open segment, -1 if none
     uint64_t segment = 0;   // id of that segment
+    int32_t source = -1;    // recorded plan input (no data yet), -1 if none
+    int8_t konst = -1;      // 0 ZERO, 1 ONES, -1 not a constant
     virtual ~StreamVal() = default;
 };
 using SV = std::shared_ptr<StreamVal>;
@@ -74,6 +76,8 @@ public:
     virtual size_t peak_bytes() const = 0;
     virtual hipStream_t stream() const = 0;
     virtual std::string stats_json() const { return "{}"; }
+    // Called before input() with the DSL name (and array index, or -1).
+    virtual void note_input(const std::string& /*name*/, int64_t /*idx*/) {}
 };
 
 struct RunResult {
@@ -88,6 +92,17 @@ struct RunResult {
 
 // Walks `prog` over `c` on `eng` (one iteration; feedback is the caller's).
 RunResult execute_program(const Program& prog, const CaseData& c, Engine& eng);
+
+// The values a walk leaves in the program's outputs, without reading them
+// back (used to record a reusable plan).
+struct RecordedOutputs {
+    std::vector<std::pair<std::string, SV>> streams;
+    std::vector<std::tuple<std::string, int64_t, SV>> array_elems;
+    struct IntOut { std::string name; PopRef pop; int64_t value = 0; };
+    std::vector<IntOut> ints;
+    int64_t op_count = 0;
+};
+RecordedOutputs record_program(const Program& prog, const CaseData& c, Engine& eng);
 
 std::unique_ptr<Engine> make_perop_engine(hipStream_t s);
 

@@ -1,12 +1,1123 @@
-// fused.cpp — placeholder until the fused engine lands.
+// fused.cpp — the fused executor: segment recording, plan cache, hipRTC
+// compilation with an on-disk code-object cache, plan launch (one prep
+// kernel + one persistent-queue kernel per partition, optionally captured
+// into a hipGraph) and the batched plan interface.
 #include "fused.h"
+
+#include "bsdata_io.h"
+#include "fused_graph.h"
+
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <unordered_map>
 
 namespace bsz {
 
-FusedOptions FusedOptions::from_env() { return FusedOptions{}; }
+uint64_t source_hash(const std::string& s);
 
-std::unique_ptr<Engine> make_fused_engine(hipStream_t, const FusedOptions&) {
-    throw RunError("fused executor not available in this build");
+namespace {
+
+int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    return (v && *v) ? std::atoi(v) : dflt;
 }
+
+}  // namespace
+
+FusedOptions FusedOptions::from_env() {
+    FusedOptions o;
+    o.chunk_rows = env_int("BSZ_CHUNK_ROWS", o.chunk_rows);
+    o.warmup_rows = env_int("BSZ_WARMUP_ROWS", o.warmup_rows);
+    o.waves_per_block = env_int("BSZ_WAVES_PER_BLOCK", o.waves_per_block);
+    o.rows_per_step = env_int("BSZ_ROWS_PER_STEP", o.rows_per_step);
+    o.max_ops = env_int("BSZ_MAX_OPS", o.max_ops);
+    o.max_live = env_int("BSZ_MAX_LIVE", o.max_live);
+    if (const char* d = std::getenv("BSZ_CACHE_DIR")) o.cache_dir = d;
+    o.verbose = env_int("BSZ_VERBOSE", 0) != 0;
+    return o;
+}
+
+FusedOptions FusedOptions::from_json(const std::string& text) {
+    FusedOptions o = from_env();
+    if (text.empty()) return o;
+    Json j = json_parse(text);
+    auto get = [&](const char* k, int& dst) {
+        if (const Json* v = j.get(k)) dst = (int)v->as_int(dst);
+    };
+    get("chunk_rows", o.chunk_rows);
+    get("warmup_rows", o.warmup_rows);
+    get("waves_per_block", o.waves_per_block);
+    get("rows_per_step", o.rows_per_step);
+    get("max_ops", o.max_ops);
+    get("max_live", o.max_live);
+    if (const Json* v = j.get("cache_dir")) o.cache_dir = v->s;
+    return o;
+}
+
+PlanOptions to_plan_options(const FusedOptions& f) {
+    PlanOptions p;
+    p.chunk_rows = std::max(1, f.chunk_rows);
+    p.warmup_rows = std::max(1, f.warmup_rows);
+    p.waves_per_block = std::min(16, std::max(1, f.waves_per_block));
+    p.rows_per_step = std::max(1, std::min(f.rows_per_step, p.chunk_rows));
+    p.max_ops = std::max(1, f.max_ops);
+    p.max_live = std::max(4, f.max_live);
+    return p;
+}
+
+// ════════════════════════════════════════════════════════════════════════
+// hipRTC
+// ════════════════════════════════════════════════════════════════════════
+
+std::string default_arch() {
+    if (const char* a = std::getenv("BSZ_OFFLOAD_ARCH")) return a;
+    return "gfx950";
+}
+
+std::vector<char> compile_module(const std::string& src, const std::string& arch, std::string* log) {
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "bsz_fused.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        throw RunError("hiprtcCreateProgram failed");
+    std::string archopt = "--offload-arch=" + arch;
+    const char* opts[] = {archopt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    size_t logsz = 0;
+    hiprtcGetProgramLogSize(prog, &logsz);
+    std::string lg(logsz, '\0');
+    if (logsz) hiprtcGetProgramLog(prog, &lg[0]);
+    if (log) *log = lg;
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        throw RunError("hipRTC compile failed: " + lg.substr(0, 4000));
+    }
+    size_t sz = 0;
+    hiprtcGetCodeSize(prog, &sz);
+    std::vector<char> code(sz);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return code;
+}
+
+namespace {
+
+std::string cache_root(const std::string& configured) {
+    if (!configured.empty()) return configured;
+    if (const char* d = std::getenv("BSZ_CACHE_DIR")) return d;
+    if (const char* h = std::getenv("HOME")) return std::string(h) + "/.cache/bsz_hip";
+    return "/tmp/bsz_hip_cache";
+}
+
+void mkdirs(const std::string& path) {
+    std::string cur;
+    for (size_t i = 0; i < path.size(); ++i) {
+        cur += path[i];
+        if (path[i] == '/' && cur.size() > 1) mkdir(cur.c_str(), 0755);
+    }
+    mkdir(path.c_str(), 0755);
+}
+
+struct Module {
+    hipModule_t mod = nullptr;
+    std::vector<hipFunction_t> fns;
+    ~Module() {
+        if (mod) (void)hipModuleUnload(mod);
+    }
+};
+using ModuleP = std::shared_ptr<Module>;
+
+// Code objects cached in memory by source hash and on disk, so a plan compiled
+// once (in this process or an earlier one) is loaded without hipRTC.
+class Jit {
+public:
+    explicit Jit(std::string dir) : dir_(cache_root(dir)) {}
+
+    std::vector<char> code_for(const std::string& src, double* compile_ms) {
+        const std::string arch = default_arch();
+        const uint64_t h = source_hash(src + "|" + arch);
+        char name[64];
+        std::snprintf(name, sizeof name, "/%016llx.co", (unsigned long long)h);
+        const std::string path = dir_ + name;
+        if (const char* dump = std::getenv("BSZ_DUMP_DIR")) {
+            mkdirs(dump);
+            std::ofstream f(std::string(dump) + std::string(name).substr(0, 17) + ".hip");
+            f << src;
+        }
+        {
+            std::ifstream f(path, std::ios::binary);
+            if (f) {
+                std::vector<char> code((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+                if (!code.empty()) return code;
+            }
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<char> code = compile_module(src, arch, nullptr);
+        if (compile_ms) *compile_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        mkdirs(dir_);
+        const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+        {
+            std::ofstream f(tmp, std::ios::binary);
+            f.write(code.data(), (std::streamsize)code.size());
+        }
+        std::rename(tmp.c_str(), path.c_str());
+        return code;
+    }
+
+    ModuleP load(const PlanSpec& p, double* compile_ms) {
+        const uint64_t h = source_hash(p.module_source);
+        auto it = loaded_.find(h);
+        if (it != loaded_.end()) return it->second;
+        std::vector<char> code = code_for(p.module_source, compile_ms);
+        auto m = std::make_shared<Module>();
+        BSZ_HIP(hipModuleLoadData(&m->mod, code.data()));
+        for (const KernelSpec& k : p.kernels) {
+            hipFunction_t f;
+            BSZ_HIP(hipModuleGetFunction(&f, m->mod, k.name.c_str()));
+            m->fns.push_back(f);
+        }
+        loaded_[h] = m;
+        return m;
+    }
+
+private:
+    std::string dir_;
+    std::unordered_map<uint64_t, ModuleP> loaded_;
+};
+
+// ════════════════════════════════════════════════════════════════════════
+// Plan runtime: device tables and state for one plan over a set of instances
+// ════════════════════════════════════════════════════════════════════════
+
+struct Arena {
+    size_t bytes = 0;
+    size_t take(size_t n) {
+        size_t o = (bytes + 255) & ~size_t(255);
+        bytes = o + n;
+        return o;
+    }
+};
+
+class PlanRT {
+public:
+    std::shared_ptr<PlanSpec> spec;
+    ModuleP mod;
+    uint32_t n_inst = 0, n_chunks = 0;
+    uint64_t chunk_rows = 16;
+
+    ~PlanRT() { release(); }
+
+    void release() {
+        if (graph_) { (void)hipGraphExecDestroy(graph_); graph_ = nullptr; }
+        if (dev_) { (void)hipFree(dev_); dev_ = nullptr; }
+        dev_bytes_ = 0;
+    }
+
+    // Lays out the device arena for `geoms` (one per instance).
+    void configure(const std::vector<Geom>& geoms, hipStream_t s) {
+        const PlanSpec& P = *spec;
+        const int U = std::max(1, P.opt.rows_per_step);
+        chunk_rows = (uint64_t)std::max(U, (P.opt.chunk_rows / U) * U);
+        n_inst = (uint32_t)geoms.size();
+        inst_h_.assign(n_inst, InstDesc{});
+        uint64_t nc = 0;
+        for (uint32_t i = 0; i < n_inst; ++i) {
+            InstDesc& d = inst_h_[i];
+            d.rows = geoms[i].rows;
+            d.lw = geoms[i].lw;
+            d.lastmask = geoms[i].lastmask;
+            d.chunk_begin = (uint32_t)nc;
+            d.n_chunks = (uint32_t)((d.rows + chunk_rows - 1) / chunk_rows);
+            nc += d.n_chunks;
+        }
+        if (nc >= 0xffffffffull) throw RunError("too many chunks in one launch");
+        n_chunks = (uint32_t)nc;
+        chunk_inst_h_.clear();
+        if (n_inst > 1) {
+            chunk_inst_h_.resize(n_chunks);
+            for (uint32_t i = 0; i < n_inst; ++i)
+                for (uint32_t c = 0; c < inst_h_[i].n_chunks; ++c) chunk_inst_h_[inst_h_[i].chunk_begin + c] = i;
+        }
+        Arena a;
+        const size_t K = P.kernels.size();
+        kres_.assign(K, KRes{});
+        size_t total_pop = 0;
+        off_inst_ = a.take(sizeof(InstDesc) * n_inst);
+        off_chunk_inst_ = n_inst > 1 ? a.take(4ull * n_chunks) : 0;
+        off_ctl_ = a.take(16ull * std::max<size_t>(1, K));
+        for (size_t k = 0; k < K; ++k) total_pop += P.kernels[k].pops.size() * n_inst;
+        off_pop_ = a.take(8ull * std::max<size_t>(1, total_pop));
+        size_t pop_cursor = 0;
+        for (size_t k = 0; k < K; ++k) {
+            const KernelSpec& ks = P.kernels[k];
+            KRes& r = kres_[k];
+            r.nin = ks.inputs.size();
+            r.nout = ks.outputs.size();
+            r.npop = ks.pops.size();
+            r.nst = ks.n_state;
+            r.in_tab = a.take(8ull * std::max<size_t>(1, r.nin * n_inst));
+            r.out_tab = a.take(8ull * std::max<size_t>(1, r.nout * n_inst));
+            r.pop = off_pop_ + 8ull * pop_cursor;
+            pop_cursor += (size_t)r.npop * n_inst;
+            if (r.nst) {
+                r.st_u = a.take(8ull * n_chunks * r.nst);
+                r.st_t = a.take(8ull * n_chunks * r.nst);
+                r.flag = a.take(4ull * n_chunks);
+                r.partial = a.take(4ull * n_chunks * std::max(1, r.npop));
+                r.mism = a.take(4ull * n_chunks);
+            }
+        }
+        total_pop_ = total_pop;
+        if (a.bytes > dev_bytes_) {
+            if (graph_) { (void)hipGraphExecDestroy(graph_); graph_ = nullptr; }
+            if (dev_) { BSZ_HIP(hipStreamSynchronize(s)); BSZ_HIP(hipFree(dev_)); }
+            BSZ_HIP(hipMalloc(&dev_, a.bytes));
+            BSZ_HIP(hipMemsetAsync(dev_, 0, a.bytes, s));
+            dev_bytes_ = a.bytes;
+        }
+        in_tab_h_.assign(K, {});
+        out_tab_h_.assign(K, {});
+        for (size_t k = 0; k < K; ++k) {
+            in_tab_h_[k].assign((size_t)kres_[k].nin * n_inst, nullptr);
+            out_tab_h_[k].assign((size_t)kres_[k].nout * n_inst, nullptr);
+        }
+        BSZ_HIP(hipMemcpyAsync(ptr(off_inst_), inst_h_.data(), sizeof(InstDesc) * n_inst, hipMemcpyHostToDevice, s));
+        if (n_inst > 1)
+            BSZ_HIP(hipMemcpyAsync(ptr(off_chunk_inst_), chunk_inst_h_.data(), 4ull * n_chunks, hipMemcpyHostToDevice, s));
+        tables_dirty_ = true;
+    }
+
+    void set_in(size_t k, uint32_t inst, size_t j, const uint64_t* p) {
+        in_tab_h_[k][(size_t)inst * kres_[k].nin + j] = p;
+        tables_dirty_ = true;
+    }
+    void set_out(size_t k, uint32_t inst, size_t j, uint64_t* p) {
+        out_tab_h_[k][(size_t)inst * kres_[k].nout + j] = p;
+        tables_dirty_ = true;
+    }
+
+    void upload_tables(hipStream_t s) {
+        if (!tables_dirty_) return;
+        for (size_t k = 0; k < kres_.size(); ++k) {
+            if (kres_[k].nin) BSZ_HIP(hipMemcpyAsync(ptr(kres_[k].in_tab), in_tab_h_[k].data(), 8ull * in_tab_h_[k].size(), hipMemcpyHostToDevice, s));
+            if (kres_[k].nout) BSZ_HIP(hipMemcpyAsync(ptr(kres_[k].out_tab), out_tab_h_[k].data(), 8ull * out_tab_h_[k].size(), hipMemcpyHostToDevice, s));
+        }
+        BSZ_HIP(hipStreamSynchronize(s));  // host tables may change after this call
+        tables_dirty_ = false;
+    }
+
+    // prep + every kernel, asynchronously on s.
+    void launch(hipStream_t s) {
+        const PlanSpec& P = *spec;
+        launch_plan_prep(s, (unsigned*)ptr(off_ctl_), (uint32_t)P.kernels.size(), (unsigned long long*)ptr(off_pop_),
+                         total_pop_);
+        const int wpb = P.opt.waves_per_block;
+        const uint32_t blocks = (n_chunks + wpb - 1) / wpb;
+        for (size_t k = 0; k < P.kernels.size(); ++k) {
+            const KRes& r = kres_[k];
+            KArgs A{};
+            A.in = (const uint64_t* const*)ptr(r.in_tab);
+            A.out = (uint64_t* const*)ptr(r.out_tab);
+            A.pop_acc = (unsigned long long*)ptr(r.pop);
+            A.pop_out = A.pop_acc;
+            A.inst = (const InstDesc*)ptr(off_inst_);
+            A.chunk_inst = n_inst > 1 ? (const uint32_t*)ptr(off_chunk_inst_) : nullptr;
+            A.st_u = r.nst ? (unsigned long long*)ptr(r.st_u) : nullptr;
+            A.st_t = r.nst ? (unsigned long long*)ptr(r.st_t) : nullptr;
+            A.st_flag = r.nst ? (unsigned*)ptr(r.flag) : nullptr;
+            A.partial = r.nst ? (unsigned*)ptr(r.partial) : nullptr;
+            A.mismatch = r.nst ? (unsigned*)ptr(r.mism) : nullptr;
+            A.ctl = (unsigned*)ptr(off_ctl_ + 16ull * k);
+            A.n_chunks = n_chunks;
+            A.n_inst = n_inst;
+            void* params[] = {&A};
+            BSZ_HIP(hipModuleLaunchKernel(mod->fns[k], blocks, 1, 1, 64 * wpb, 1, 1, 0, s, params, nullptr));
+        }
+        launches_ += 1 + (int64_t)P.kernels.size();
+    }
+
+    void capture(hipStream_t s) {
+        if (graph_) { (void)hipGraphExecDestroy(graph_); graph_ = nullptr; }
+        hipGraph_t g;
+        BSZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        launch(s);
+        BSZ_HIP(hipStreamEndCapture(s, &g));
+        BSZ_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+    }
+
+    void replay(hipStream_t s) {
+        if (!graph_) throw RunError("plan has no captured graph");
+        BSZ_HIP(hipGraphLaunch(graph_, s));
+        launches_ += 1 + (int64_t)spec->kernels.size();
+    }
+
+    // popcount of SegGraph pop `p` for instance `inst`, after a run (sync)
+    void read_pops(hipStream_t s, std::vector<unsigned long long>& out) {
+        out.assign(total_pop_, 0);
+        if (total_pop_)
+            BSZ_HIP(hipMemcpyAsync(out.data(), ptr(off_pop_), 8ull * total_pop_, hipMemcpyDeviceToHost, s));
+        BSZ_HIP(hipStreamSynchronize(s));
+    }
+    // index into read_pops() output of graph pop p for instance i
+    size_t pop_index(size_t p, uint32_t inst) const {
+        size_t base = 0;
+        for (size_t k = 0; k < spec->kernels.size(); ++k) {
+            const KernelSpec& ks = spec->kernels[k];
+            for (size_t j = 0; j < ks.pops.size(); ++j)
+                if ((size_t)ks.pops[j] == p) return base + (size_t)inst * ks.pops.size() + j;
+            base += ks.pops.size() * n_inst;
+        }
+        throw RunError("popcount not in plan");
+    }
+    // mismatching chunks seen by kernel k in the last run (sync)
+    uint32_t mismatches(hipStream_t s, size_t k) {
+        uint32_t v = 0;
+        BSZ_HIP(hipMemcpyAsync(&v, ptr(off_ctl_ + 16ull * k + 12), 4, hipMemcpyDeviceToHost, s));
+        BSZ_HIP(hipStreamSynchronize(s));
+        return v;
+    }
+    int64_t launches() const { return launches_; }
+    size_t device_bytes() const { return dev_bytes_; }
+
+private:
+    struct KRes {
+        size_t nin = 0, nout = 0;
+        int npop = 0, nst = 0;
+        size_t in_tab = 0, out_tab = 0, pop = 0, st_u = 0, st_t = 0, flag = 0, partial = 0, mism = 0;
+    };
+    void* dev_ = nullptr;
+    size_t dev_bytes_ = 0;
+    size_t off_inst_ = 0, off_chunk_inst_ = 0, off_ctl_ = 0, off_pop_ = 0, total_pop_ = 0;
+    std::vector<KRes> kres_;
+    std::vector<InstDesc> inst_h_;
+    std::vector<uint32_t> chunk_inst_h_;
+    std::vector<std::vector<const uint64_t*>> in_tab_h_;
+    std::vector<std::vector<uint64_t*>> out_tab_h_;
+    bool tables_dirty_ = true;
+    hipGraphExec_t graph_ = nullptr;
+    int64_t launches_ = 0;
+
+    char* ptr(size_t off) const { return static_cast<char*>(dev_) + off; }
+};
+
+// Binds the buffers of one instance: sources, segment outputs and slots.
+void bind_instance(PlanRT& rt, uint32_t inst, const std::vector<const uint64_t*>& sources,
+                   const std::vector<uint64_t*>& seg_out_by_node, const std::vector<uint64_t*>& slots) {
+    const PlanSpec& P = *rt.spec;
+    for (size_t k = 0; k < P.kernels.size(); ++k) {
+        const KernelSpec& ks = P.kernels[k];
+        for (size_t j = 0; j < ks.inputs.size(); ++j) {
+            const KInput& in = ks.inputs[j];
+            const uint64_t* p = in.kind == KInput::Source ? sources.at(in.id)
+                              : in.kind == KInput::Slot ? slots.at(in.id)
+                                                        : seg_out_by_node.at(in.id);
+            rt.set_in(k, inst, j, p);
+        }
+        for (size_t j = 0; j < ks.outputs.size(); ++j) {
+            uint64_t* p = ks.out_slot[j] >= 0 ? slots.at(ks.out_slot[j]) : seg_out_by_node.at(ks.outputs[j]);
+            rt.set_out(k, inst, j, p);
+        }
+    }
+}
+
+// ════════════════════════════════════════════════════════════════════════
+// FusedEngine
+// ════════════════════════════════════════════════════════════════════════
+
+struct FusedPop : PopCell {
+    int32_t node = -1;
+    uint64_t segment = 0;
+};
+
+struct CachedPlan {
+    std::shared_ptr<PlanSpec> spec;
+    ModuleP mod;
+    std::unique_ptr<PlanRT> rt;
+};
+
+class FusedEngine final : public Engine {
+public:
+    FusedEngine(hipStream_t s, const FusedOptions& opt, bool recording)
+        : s_(s), fopt_(opt), popt_(to_plan_options(opt)), jit_(opt.cache_dir), recording_(recording) {}
+
+    ~FusedEngine() override {
+        if (!recording_) (void)hipStreamSynchronize(s_);
+    }
+
+    // ── Engine ──
+    void begin(const Geom& g) override {
+        if (!recording_) flush();
+        g_ = g;
+        zero_.reset();
+        ones_.reset();
+        launches0_ = launches_;
+    }
+
+    SV constant(bool ones) override {
+        SV& c = ones ? ones_ : zero_;
+        if (!c) {
+            c = std::make_shared<StreamVal>();
+            c->konst = ones ? 1 : 0;
+        }
+        return c;
+    }
+
+    SV input(const uint64_t* host, uint64_t n) override {
+        auto v = std::make_shared<StreamVal>();
+        if (recording_) {
+            v->source = (int32_t)rec_sources_.size();
+            rec_sources_.push_back(pending_input_);
+            return v;
+        }
+        v->buf = std::make_shared<DevBuf>(&pool_, g_.bytes());
+        upload_stream(s_, v->buf->ptr, host, n, g_);
+        BSZ_HIP(hipStreamSynchronize(s_));
+        return v;
+    }
+
+    void note_input(const std::string& name, int64_t idx) override { pending_input_ = {name, idx}; }
+
+    SV op_not(const SV& a) override { return make(NOp::Not, node_of(a), -1, 0); }
+    SV op_bin(BinOp op, const SV& a, const SV& b) override {
+        NOp o = op == BinOp::And ? NOp::And : op == BinOp::Or ? NOp::Or : op == BinOp::Xor ? NOp::Xor : NOp::Add;
+        int32_t x = node_of(a), y = node_of(b);
+        return make(o, x, y, 0);
+    }
+    SV op_shift(bool left, const SV& a, uint64_t k) override {
+        return make(left ? NOp::Shl : NOp::Shr, node_of(a), -1, k);
+    }
+
+    PopRef popcount(const SV& a) override {
+        auto p = std::make_shared<FusedPop>();
+        p->node = node_of(a);
+        p->segment = seg_;
+        pops_.push_back(p);
+        return p;
+    }
+
+    int64_t resolve(const PopRef& p) override {
+        if (!p->ready) {
+            if (recording_) throw RunError("program reads a popcount on the host; it cannot be recorded as one plan");
+            flush();
+            if (!p->ready) throw RunError("internal: popcount not produced by flush");
+        }
+        return p->value;
+    }
+
+    bool is_nonzero(const SV& a) override {
+        if (recording_) throw RunError("program branches on stream contents; it cannot be recorded as one plan");
+        flush();
+        if (a->konst >= 0) return a->konst == 1 && g_.bitlength > 0;
+        if (!slot_) BSZ_HIP(hipMalloc(&slot_, 8));
+        launch_zero_u64(s_, slot_, 1);
+        launch_nonzero(s_, slot_, a->buf->ptr, g_);
+        unsigned long long v = 0;
+        BSZ_HIP(hipMemcpyAsync(&v, slot_, 8, hipMemcpyDeviceToHost, s_));
+        BSZ_HIP(hipStreamSynchronize(s_));
+        launches_ += 2;
+        return v != 0;
+    }
+
+    void download(const SV& a, uint64_t* host) override {
+        flush();
+        if (a->konst >= 0) {
+            for (uint64_t i = 0; i < g_.lw; ++i) host[i] = a->konst ? ~0ull : 0ull;
+            if (a->konst && g_.lw) host[g_.lw - 1] &= g_.lastmask;
+            return;
+        }
+        BSZ_HIP(hipMemcpyAsync(host, a->buf->ptr, g_.lw * 8, hipMemcpyDeviceToHost, s_));
+        BSZ_HIP(hipStreamSynchronize(s_));
+    }
+
+    void flush() override;
+    void sync() override { BSZ_HIP(hipStreamSynchronize(s_)); }
+    int64_t n_kernel_launches() const override { return launches_; }
+    int64_t n_mallocs() const override { return pool_.n_mallocs(); }
+    size_t peak_bytes() const override { return pool_.peak_bytes(); }
+    hipStream_t stream() const override { return s_; }
+    std::string stats_json() const override {
+        std::ostringstream o;
+        o << "{\"executor\":\"fused\",\"flushes\":" << flushes_ << ",\"plans\":" << plans_.size()
+          << ",\"kernels\":" << kernels_run_ << ",\"compile_ms\":" << compile_ms_
+          << ",\"fixup_chunks\":" << fixups_ << "}";
+        return o.str();
+    }
+
+    // ── recording (plan interface) ──
+    SegGraph take_graph(std::vector<SV>& outputs_in_order, std::vector<int32_t>& out_nodes,
+                        std::vector<std::shared_ptr<FusedPop>>& pops) {
+        SegGraph g;
+        g.nodes = nodes_;
+        g.n_sources = (int32_t)rec_sources_.size();
+        for (const SV& v : outputs_in_order) {
+            int32_t n = node_of(v);
+            out_nodes.push_back(n);
+        }
+        std::vector<char> seen(g.nodes.size(), 0);
+        for (int32_t n : out_nodes)
+            if (!seen[n]) { seen[n] = 1; g.outputs.push_back(n); }
+        for (auto& p : pops_) { g.pops.push_back(p->node); pops.push_back(p); }
+        g.nodes = nodes_;  // node_of() above may have added input/constant nodes
+        return g;
+    }
+    const std::vector<std::pair<std::string, int64_t>>& recorded_sources() const { return rec_sources_; }
+    Jit& jit() { return jit_; }
+    const PlanOptions& plan_options() const { return popt_; }
+    DevPool& pool() { return pool_; }
+    double& compile_ms() { return compile_ms_; }
+
+private:
+    hipStream_t s_;
+    FusedOptions fopt_;
+    PlanOptions popt_;
+    Jit jit_;
+    bool recording_;
+    Geom g_;
+    DevPool pool_;
+    SV zero_, ones_;
+    unsigned long long* slot_ = nullptr;
+
+    uint64_t seg_ = 1;
+    std::vector<GNode> nodes_;
+    std::vector<std::weak_ptr<StreamVal>> node_val_;
+    std::vector<std::shared_ptr<FusedPop>> pops_;
+    std::vector<DevBufP> sources_;
+    std::unordered_map<const void*, int32_t> src_node_;
+    int32_t zero_node_ = -1, ones_node_ = -1;
+    std::vector<std::pair<std::string, int64_t>> rec_sources_;
+    std::pair<std::string, int64_t> pending_input_{"", -1};
+    std::unordered_map<std::string, CachedPlan> plans_;
+
+    int64_t launches_ = 0, launches0_ = 0, flushes_ = 0, kernels_run_ = 0, fixups_ = 0;
+    double compile_ms_ = 0;
+
+    int32_t add_node(const GNode& n, const SV& v) {
+        nodes_.push_back(n);
+        node_val_.emplace_back(v);
+        return (int32_t)nodes_.size() - 1;
+    }
+
+    int32_t node_of(const SV& v) {
+        if (v->node >= 0 && v->segment == seg_) return v->node;
+        if (v->konst >= 0) {
+            int32_t& id = v->konst ? ones_node_ : zero_node_;
+            if (id < 0) {
+                GNode n;
+                n.op = v->konst ? NOp::Ones : NOp::Zero;
+                id = add_node(n, nullptr);
+            }
+            return id;
+        }
+        if (recording_ && v->source >= 0) {
+            GNode n;
+            n.op = NOp::Input;
+            n.src = v->source;
+            int32_t id = add_node(n, v);
+            v->node = id;
+            v->segment = seg_;
+            return id;
+        }
+        if (!v->buf) throw RunError("internal: stream value without data");
+        auto it = src_node_.find(v->buf->ptr);
+        if (it != src_node_.end()) return it->second;
+        GNode n;
+        n.op = NOp::Input;
+        n.src = (int32_t)sources_.size();
+        sources_.push_back(v->buf);
+        int32_t id = add_node(n, v);
+        src_node_[v->buf->ptr] = id;
+        return id;
+    }
+
+    SV make(NOp op, int32_t a, int32_t b, uint64_t k) {
+        auto v = std::make_shared<StreamVal>();
+        GNode n;
+        n.op = op;
+        n.a = a;
+        n.b = b;
+        n.k = k;
+        v->node = add_node(n, v);
+        v->segment = seg_;
+        return v;
+    }
+
+    void reset_segment() {
+        nodes_.clear();
+        node_val_.clear();
+        pops_.clear();
+        sources_.clear();
+        src_node_.clear();
+        zero_node_ = ones_node_ = -1;
+        ++seg_;
+    }
+
+public:
+    CachedPlan& plan_for(const SegGraph& g) {
+        std::string key = g.key() + "#" + popt_.key();
+        auto it = plans_.find(key);
+        if (it != plans_.end()) return it->second;
+        CachedPlan cp;
+        cp.spec = std::make_shared<PlanSpec>(build_plan(g, popt_));
+        if (!cp.spec->kernels.empty()) cp.mod = jit_.load(*cp.spec, &compile_ms_);
+        cp.rt = std::make_unique<PlanRT>();
+        cp.rt->spec = cp.spec;
+        cp.rt->mod = cp.mod;
+        return plans_.emplace(key, std::move(cp)).first->second;
+    }
+};
+
+void FusedEngine::flush() {
+    if (recording_) return;
+    if (nodes_.empty() && pops_.empty()) return;
+    ++flushes_;
+    SegGraph g;
+    g.nodes = nodes_;
+    g.n_sources = (int32_t)sources_.size();
+    std::vector<SV> out_vals;
+    for (size_t i = 0; i < nodes_.size(); ++i) {
+        if (nodes_[i].op == NOp::Input || nodes_[i].op == NOp::Zero || nodes_[i].op == NOp::Ones) continue;
+        if (SV v = node_val_[i].lock()) {
+            g.outputs.push_back((int32_t)i);
+            out_vals.push_back(v);
+        }
+    }
+    for (auto& p : pops_) g.pops.push_back(p->node);
+    if (!g.outputs.empty() || !g.pops.empty()) {
+        CachedPlan& cp = plan_for(g);
+        PlanRT& rt = *cp.rt;
+        rt.configure({g_}, s_);
+        std::vector<const uint64_t*> src(sources_.size());
+        for (size_t i = 0; i < sources_.size(); ++i) src[i] = sources_[i]->ptr;
+        std::vector<uint64_t*> seg_out(nodes_.size(), nullptr);
+        std::vector<DevBufP> out_bufs;
+        for (size_t i = 0; i < g.outputs.size(); ++i) {
+            auto b = std::make_shared<DevBuf>(&pool_, g_.bytes());
+            seg_out[g.outputs[i]] = b->ptr;
+            out_bufs.push_back(b);
+        }
+        std::vector<DevBufP> slot_bufs;
+        std::vector<uint64_t*> slots;
+        for (int i = 0; i < cp.spec->n_slots; ++i) {
+            slot_bufs.push_back(std::make_shared<DevBuf>(&pool_, g_.bytes()));
+            slots.push_back(slot_bufs.back()->ptr);
+        }
+        if (!cp.spec->kernels.empty()) {
+            bind_instance(rt, 0, src, seg_out, slots);
+            rt.upload_tables(s_);
+            rt.launch(s_);
+            launches_ += 1 + (int64_t)cp.spec->kernels.size();
+            kernels_run_ += (int64_t)cp.spec->kernels.size();
+        }
+        if (!pops_.empty()) {
+            std::vector<unsigned long long> vals;
+            rt.read_pops(s_, vals);
+            for (size_t p = 0; p < pops_.size(); ++p) {
+                pops_[p]->value = g_.bitlength > 0 ? (int64_t)vals[rt.pop_index(p, 0)] : 0;
+                pops_[p]->ready = true;
+            }
+        }
+        if (fopt_.verbose) {
+            for (size_t k = 0; k < cp.spec->kernels.size(); ++k)
+                if (cp.spec->kernels[k].n_state) fixups_ += rt.mismatches(s_, k);
+        }
+        for (size_t i = 0; i < out_vals.size(); ++i) {
+            out_vals[i]->buf = out_bufs[i];
+            out_vals[i]->node = -1;
+        }
+    }
+    for (size_t i = 0; i < node_val_.size(); ++i)
+        if (SV v = node_val_[i].lock()) v->node = -1;
+    reset_segment();
+}
+
+}  // namespace
+
+std::unique_ptr<Engine> make_fused_engine(hipStream_t s, const FusedOptions& opt) {
+    return std::make_unique<FusedEngine>(s, opt, false);
+}
+
+// ════════════════════════════════════════════════════════════════════════
+// Recorded plans (the batched interface behind bsz_plan_*)
+// ════════════════════════════════════════════════════════════════════════
+
+struct FusedPlan::Impl {
+    hipStream_t s = nullptr;
+    FusedOptions fopt;
+    std::unique_ptr<FusedEngine> rec;       // the recording engine (owns the JIT and the pool)
+    std::shared_ptr<PlanSpec> spec;
+    ModuleP mod;
+    std::unique_ptr<PlanRT> rt;
+    std::vector<std::pair<std::string, int64_t>> sources;   // per source: (name, idx)
+    struct Out { std::string name; int64_t idx; int kind; int32_t ref; };  // kind 0 node, 1 source alias, 2 const
+    std::vector<Out> outs;
+    struct IntOut { std::string name; int32_t pop; int64_t value; };
+    std::vector<IntOut> ints;
+    int64_t op_count = 0;
+    std::vector<Geom> geoms;
+    std::vector<std::vector<const uint64_t*>> bound;       // [inst][source]
+    std::vector<std::vector<DevBufP>> out_bufs;             // [inst][g.outputs index]
+    std::vector<std::vector<DevBufP>> slot_bufs;            // [inst][slot]
+    std::vector<std::vector<DevBufP>> const_bufs;           // [inst][outs index] for constant outputs
+    std::vector<unsigned long long> pops_host;
+    bool bound_dirty = true;
+
+    // one kernel per DSL op over the same recorded DAG (the plain comparator)
+    int executor = 1;                                       // 1 fused, 0 per-op
+    std::vector<int32_t> perop_order;                       // needed computed nodes, in order
+    std::vector<int32_t> perop_buf;                         // node -> scratch buffer index (-1: output/source)
+    int perop_nbufs = 0;
+    struct PerOpInst {
+        std::vector<DevBufP> scratch;
+        DevBufP zero, ones;
+        void* add_state = nullptr;
+        unsigned long long* pops = nullptr;
+    };
+    std::vector<PerOpInst> perop;
+    std::vector<unsigned long long> perop_pops_host;
+
+    ~Impl() {
+        for (auto& p : perop) {
+            if (p.add_state) (void)hipFree(p.add_state);
+            if (p.pops) (void)hipFree(p.pops);
+        }
+    }
+
+    void plan_perop() {
+        const SegGraph& g = spec->g;
+        const int32_t N = (int32_t)g.nodes.size();
+        std::vector<char> need(N, 0), is_out(N, 0);
+        for (int32_t o : g.outputs) { need[o] = 1; is_out[o] = 1; }
+        for (int32_t p : g.pops) need[p] = 1;
+        auto ar = [](NOp op) {
+            return (op == NOp::Not || op == NOp::Shl || op == NOp::Shr) ? 1
+                 : (op == NOp::And || op == NOp::Or || op == NOp::Xor || op == NOp::Add) ? 2 : 0;
+        };
+        for (int32_t n = N - 1; n >= 0; --n) {
+            if (!need[n]) continue;
+            if (ar(g.nodes[n].op) >= 1) need[g.nodes[n].a] = 1;
+            if (ar(g.nodes[n].op) >= 2) need[g.nodes[n].b] = 1;
+        }
+        std::vector<int32_t> last(N, -1);
+        for (int32_t n = 0; n < N; ++n) {
+            if (!need[n]) continue;
+            if (ar(g.nodes[n].op) >= 1) last[g.nodes[n].a] = n;
+            if (ar(g.nodes[n].op) >= 2) last[g.nodes[n].b] = std::max(last[g.nodes[n].b], n);
+        }
+        for (int32_t p : g.pops) last[p] = N;
+        perop_buf.assign(N, -1);
+        perop_order.clear();
+        std::vector<int> free_list;
+        int nb = 0;
+        for (int32_t n = 0; n < N; ++n) {
+            if (!need[n] || ar(g.nodes[n].op) == 0) continue;
+            perop_order.push_back(n);
+            if (!is_out[n]) {
+                int b;
+                if (!free_list.empty()) { b = free_list.back(); free_list.pop_back(); }
+                else b = nb++;
+                perop_buf[n] = b;
+            }
+            // operands whose last use is this node hand their buffer back (after
+            // this node's own buffer was taken, so dst never aliases an operand)
+            const GNode& x = g.nodes[n];
+            const int32_t ops2[2] = {x.a, (ar(x.op) >= 2 && x.b != x.a) ? x.b : -1};
+            for (int32_t a : ops2)
+                if (a >= 0 && last[a] == n && perop_buf[a] >= 0) free_list.push_back(perop_buf[a]);
+        }
+        perop_nbufs = nb;
+    }
+};
+
+FusedPlan::FusedPlan() : impl_(new Impl) {}
+FusedPlan::~FusedPlan() {
+    if (impl_ && impl_->s) (void)hipStreamSynchronize(impl_->s);
+    if (graph_) (void)hipGraphExecDestroy((hipGraphExec_t)graph_);
+}
+
+std::unique_ptr<FusedPlan> FusedPlan::record(hipStream_t s, const Program& prog, const CaseData& shape,
+                                             const FusedOptions& opt, bool compile) {
+    std::unique_ptr<FusedPlan> fp(new FusedPlan);
+    Impl& I = *fp->impl_;
+    I.s = s;
+    I.fopt = opt;
+    I.rec = std::make_unique<FusedEngine>(s, opt, true);
+    RecordedOutputs ro = record_program(prog, shape, *I.rec);
+    I.op_count = ro.op_count;
+    std::vector<SV> ovals;
+    for (auto& kv : ro.streams) ovals.push_back(kv.second);
+    for (auto& t : ro.array_elems) ovals.push_back(std::get<2>(t));
+    std::vector<int32_t> out_nodes;
+    std::vector<std::shared_ptr<FusedPop>> pops;
+    SegGraph g = I.rec->take_graph(ovals, out_nodes, pops);
+    I.sources = I.rec->recorded_sources();
+    // outputs that are inputs or constants need no kernel
+    std::vector<int32_t> computed;
+    std::vector<char> seen(g.nodes.size(), 0);
+    auto out_kind = [&](int32_t n) {
+        if (g.nodes[n].op == NOp::Input) return 1;
+        if (g.nodes[n].op == NOp::Zero || g.nodes[n].op == NOp::Ones) return 2;
+        return 0;
+    };
+    size_t oi = 0;
+    auto add_out = [&](const std::string& name, int64_t idx) {
+        int32_t n = out_nodes[oi++];
+        Impl::Out o{name, idx, out_kind(n), n};
+        if (o.kind == 0 && !seen[n]) { seen[n] = 1; computed.push_back(n); }
+        I.outs.push_back(o);
+    };
+    for (auto& kv : ro.streams) add_out(kv.first, -1);
+    for (auto& t : ro.array_elems) add_out(std::get<0>(t), std::get<1>(t));
+    g.outputs = computed;
+    for (auto& io : ro.ints) {
+        Impl::IntOut x{io.name, -1, io.value};
+        if (io.pop) {
+            for (size_t p = 0; p < pops.size(); ++p)
+                if (pops[p].get() == io.pop.get()) x.pop = (int32_t)p;
+            if (x.pop < 0) throw RunError("internal: int output popcount not recorded");
+        }
+        I.ints.push_back(x);
+    }
+    I.spec = std::make_shared<PlanSpec>(build_plan(g, to_plan_options(opt)));
+    if (compile && !I.spec->kernels.empty()) {
+        std::vector<char> code = I.rec->jit().code_for(I.spec->module_source, &I.rec->compile_ms());
+        (void)code;
+    }
+    return fp;
+}
+
+void FusedPlan::load() {
+    Impl& I = *impl_;
+    if (!I.mod && !I.spec->kernels.empty()) I.mod = I.rec->jit().load(*I.spec, &I.rec->compile_ms());
+    if (!I.rt) {
+        I.rt = std::make_unique<PlanRT>();
+        I.rt->spec = I.spec;
+        I.rt->mod = I.mod;
+    }
+}
+
+void FusedPlan::set_instances(const std::vector<int64_t>& bitlengths) {
+    Impl& I = *impl_;
+    load();
+    I.geoms.clear();
+    for (int64_t bl : bitlengths) I.geoms.push_back(Geom::make(bl, 1));
+    I.rt->configure(I.geoms, I.s);
+    const size_t n = I.geoms.size();
+    I.bound.assign(n, std::vector<const uint64_t*>(I.sources.size(), nullptr));
+    I.out_bufs.assign(n, {});
+    I.slot_bufs.assign(n, {});
+    I.const_bufs.assign(n, std::vector<DevBufP>(I.outs.size()));
+    for (size_t i = 0; i < n; ++i) {
+        for (size_t o = 0; o < I.outs.size(); ++o) {
+            if (I.outs[o].kind != 2) continue;
+            auto b = std::make_shared<DevBuf>(&I.rec->pool(), I.geoms[i].bytes());
+            launch_fill(I.s, b->ptr, I.spec->g.nodes[I.outs[o].ref].op == NOp::Ones, I.geoms[i]);
+            I.const_bufs[i][o] = b;
+        }
+        for (size_t o = 0; o < I.spec->g.outputs.size(); ++o)
+            I.out_bufs[i].push_back(std::make_shared<DevBuf>(&I.rec->pool(), I.geoms[i].bytes()));
+        for (int s = 0; s < I.spec->n_slots; ++s)
+            I.slot_bufs[i].push_back(std::make_shared<DevBuf>(&I.rec->pool(), I.geoms[i].bytes()));
+    }
+    I.bound_dirty = true;
+}
+
+int FusedPlan::n_instances() const { return (int)impl_->geoms.size(); }
+
+void FusedPlan::bind_input(int inst, const std::string& name, int64_t idx, const uint64_t* dev) {
+    Impl& I = *impl_;
+    bool hit = false;
+    for (size_t j = 0; j < I.sources.size(); ++j)
+        if (I.sources[j].first == name && I.sources[j].second == idx) { I.bound.at(inst)[j] = dev; hit = true; }
+    if (!hit) throw RunError("plan has no input " + name + (idx >= 0 ? "[" + std::to_string(idx) + "]" : ""));
+    I.bound_dirty = true;
+}
+
+uint64_t* FusedPlan::output(int inst, const std::string& name, int64_t idx) {
+    Impl& I = *impl_;
+    for (size_t oi = 0; oi < I.outs.size(); ++oi) {
+        const Impl::Out& o = I.outs[oi];
+        if (o.name != name || o.idx != idx) continue;
+        if (o.kind == 1) {
+            int32_t src = I.spec->g.nodes[o.ref].src;
+            return const_cast<uint64_t*>(I.bound.at(inst).at(src));
+        }
+        if (o.kind == 2) return I.const_bufs.at(inst).at(oi)->ptr;
+        for (size_t k = 0; k < I.spec->g.outputs.size(); ++k)
+            if (I.spec->g.outputs[k] == o.ref) return I.out_bufs.at(inst)[k]->ptr;
+    }
+    throw RunError("plan has no output " + name);
+}
+
+void FusedPlan::prepare() {
+    Impl& I = *impl_;
+    if (!I.bound_dirty) return;
+    for (size_t i = 0; i < I.geoms.size(); ++i) {
+        for (size_t j = 0; j < I.sources.size(); ++j)
+            if (!I.bound[i][j]) throw RunError("plan input " + I.sources[j].first + " not bound for instance " + std::to_string(i));
+        std::vector<uint64_t*> seg_out(I.spec->g.nodes.size(), nullptr);
+        for (size_t k = 0; k < I.spec->g.outputs.size(); ++k) seg_out[I.spec->g.outputs[k]] = I.out_bufs[i][k]->ptr;
+        std::vector<uint64_t*> slots;
+        for (auto& b : I.slot_bufs[i]) slots.push_back(b->ptr);
+        if (!I.spec->kernels.empty()) bind_instance(*I.rt, (uint32_t)i, I.bound[i], seg_out, slots);
+    }
+    I.rt->upload_tables(I.s);
+    I.bound_dirty = false;
+}
+
+void FusedPlan::set_executor(int executor) {
+    Impl& I = *impl_;
+    I.executor = executor;
+    if (executor == 0 && I.perop.size() != I.geoms.size()) {
+        I.plan_perop();
+        I.perop.assign(I.geoms.size(), {});
+        for (size_t i = 0; i < I.geoms.size(); ++i) {
+            auto& P = I.perop[i];
+            const Geom& g = I.geoms[i];
+            for (int b = 0; b < I.perop_nbufs; ++b) P.scratch.push_back(std::make_shared<DevBuf>(&I.rec->pool(), g.bytes()));
+            P.zero = std::make_shared<DevBuf>(&I.rec->pool(), g.bytes());
+            P.ones = std::make_shared<DevBuf>(&I.rec->pool(), g.bytes());
+            launch_fill(I.s, P.zero->ptr, false, g);
+            launch_fill(I.s, P.ones->ptr, true, g);
+            const size_t st = AddScanState::bytes(g);
+            BSZ_HIP(hipMalloc(&P.add_state, st));
+            BSZ_HIP(hipMemsetAsync(P.add_state, 0, st, I.s));
+            BSZ_HIP(hipMalloc(&P.pops, 8 * std::max<size_t>(1, I.spec->g.pops.size())));
+        }
+        BSZ_HIP(hipStreamSynchronize(I.s));
+    }
+}
+
+void FusedPlan::run() {
+    prepare();
+    Impl& I = *impl_;
+    if (I.executor == 1) {
+        if (!I.spec->kernels.empty()) I.rt->launch(I.s);
+        return;
+    }
+    const SegGraph& g = I.spec->g;
+    for (size_t i = 0; i < I.geoms.size(); ++i) {
+        auto& P = I.perop[i];
+        const Geom& geo = I.geoms[i];
+        std::vector<uint64_t*> seg_out(g.nodes.size(), nullptr);
+        for (size_t k = 0; k < g.outputs.size(); ++k) seg_out[g.outputs[k]] = I.out_bufs[i][k]->ptr;
+        auto ptr_of = [&](int32_t n) -> uint64_t* {
+            const GNode& x = g.nodes[n];
+            if (x.op == NOp::Input) return const_cast<uint64_t*>(I.bound[i][x.src]);
+            if (x.op == NOp::Zero) return P.zero->ptr;
+            if (x.op == NOp::Ones) return P.ones->ptr;
+            if (seg_out[n]) return seg_out[n];
+            return P.scratch.at(I.perop_buf[n])->ptr;
+        };
+        for (int32_t n : I.perop_order) {
+            const GNode& x = g.nodes[n];
+            uint64_t* d = ptr_of(n);
+            switch (x.op) {
+                case NOp::Not: launch_pointwise(I.s, PwOp::Not, d, ptr_of(x.a), nullptr, geo); break;
+                case NOp::And: launch_pointwise(I.s, PwOp::And, d, ptr_of(x.a), ptr_of(x.b), geo); break;
+                case NOp::Or: launch_pointwise(I.s, PwOp::Or, d, ptr_of(x.a), ptr_of(x.b), geo); break;
+                case NOp::Xor: launch_pointwise(I.s, PwOp::Xor, d, ptr_of(x.a), ptr_of(x.b), geo); break;
+                case NOp::Add: launch_add(I.s, d, ptr_of(x.a), ptr_of(x.b), geo, P.add_state); break;
+                case NOp::Shl: launch_shl(I.s, d, ptr_of(x.a), x.k, geo); break;
+                case NOp::Shr: launch_shr(I.s, d, ptr_of(x.a), x.k, geo); break;
+                default: break;
+            }
+        }
+        if (!g.pops.empty()) {
+            launch_zero_u64(I.s, P.pops, g.pops.size());
+            for (size_t p = 0; p < g.pops.size(); ++p) launch_popcount(I.s, P.pops + p, ptr_of(g.pops[p]), geo);
+        }
+    }
+}
+
+void FusedPlan::capture() {
+    prepare();
+    Impl& I = *impl_;
+    if (I.executor == 0) {
+        if (graph_) { (void)hipGraphExecDestroy((hipGraphExec_t)graph_); graph_ = nullptr; }
+        hipGraph_t gr;
+        hipGraphExec_t ex;
+        BSZ_HIP(hipStreamBeginCapture(I.s, hipStreamCaptureModeThreadLocal));
+        run();
+        BSZ_HIP(hipStreamEndCapture(I.s, &gr));
+        BSZ_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+        graph_ = ex;
+        return;
+    }
+    if (!I.spec->kernels.empty()) I.rt->capture(I.s);
+}
+
+void FusedPlan::replay() {
+    Impl& I = *impl_;
+    if (I.executor == 0) {
+        if (!graph_) throw RunError("plan has no captured graph");
+        BSZ_HIP(hipGraphLaunch((hipGraphExec_t)graph_, I.s));
+        return;
+    }
+    if (!I.spec->kernels.empty()) I.rt->replay(I.s);
+}
+
+void FusedPlan::prepare_now() { prepare(); }
+
+int64_t FusedPlan::read_int(int inst, const std::string& name, bool refresh) {
+    Impl& I = *impl_;
+    for (auto& io : I.ints) {
+        if (io.name != name) continue;
+        if (io.pop < 0) return io.value;
+        if (I.executor == 0) {
+            unsigned long long v = 0;
+            BSZ_HIP(hipMemcpyAsync(&v, I.perop.at(inst).pops + io.pop, 8, hipMemcpyDeviceToHost, I.s));
+            BSZ_HIP(hipStreamSynchronize(I.s));
+            return I.geoms.at(inst).bitlength <= 0 ? 0 : (int64_t)v;
+        }
+        if (refresh || I.pops_host.empty()) I.rt->read_pops(I.s, I.pops_host);
+        if (I.geoms.at(inst).bitlength <= 0) return 0;
+        return (int64_t)I.pops_host.at(I.rt->pop_index(io.pop, inst));
+    }
+    throw RunError("plan has no int output " + name);
+}
+
+uint32_t FusedPlan::mismatches(int kernel) { return impl_->rt->mismatches(impl_->s, kernel); }
+
+std::string FusedPlan::info_json() const {
+    const Impl& I = *impl_;
+    std::ostringstream o;
+    o << "{\"kernels\":[";
+    for (size_t k = 0; k < I.spec->kernels.size(); ++k) {
+        const KernelSpec& ks = I.spec->kernels[k];
+        o << (k ? "," : "") << "{\"name\":\"" << ks.name << "\",\"nodes\":" << ks.nodes.size()
+          << ",\"inputs\":" << ks.inputs.size() << ",\"outputs\":" << ks.outputs.size() << ",\"pops\":" << ks.pops.size()
+          << ",\"state_words\":" << ks.n_state << "}";
+    }
+    o << "],\"sources\":[";
+    for (size_t j = 0; j < I.sources.size(); ++j)
+        o << (j ? "," : "") << "[\"" << I.sources[j].first << "\"," << I.sources[j].second << "]";
+    o << "],\"outputs\":[";
+    for (size_t j = 0; j < I.outs.size(); ++j)
+        o << (j ? "," : "") << "[\"" << I.outs[j].name << "\"," << I.outs[j].idx << "," << I.outs[j].kind << "]";
+    o << "],\"ints\":[";
+    for (size_t j = 0; j < I.ints.size(); ++j) o << (j ? "," : "") << "\"" << I.ints[j].name << "\"";
+    o << "],\"slots\":" << I.spec->n_slots << ",\"graph_nodes\":" << I.spec->g.nodes.size()
+      << ",\"op_count\":" << I.op_count << ",\"compile_ms\":" << I.rec->compile_ms()
+      << ",\"chunk_rows\":" << I.spec->opt.chunk_rows << ",\"warmup_rows\":" << I.spec->opt.warmup_rows
+      << ",\"waves_per_block\":" << I.spec->opt.waves_per_block << ",\"rows_per_step\":" << I.spec->opt.rows_per_step
+      << ",\"max_ops\":" << I.spec->opt.max_ops << ",\"max_live\":" << I.spec->opt.max_live
+      << ",\"n_chunks\":" << (I.rt ? I.rt->n_chunks : 0) << ",\"device_bytes\":" << (I.rt ? I.rt->device_bytes() : 0)
+      << "}";
+    return o.str();
+}
+
+const std::string& FusedPlan::module_source() const { return impl_->spec->module_source; }
+int64_t FusedPlan::op_count() const { return impl_->op_count; }
+int FusedPlan::n_kernels() const { return (int)impl_->spec->kernels.size(); }
 
 }  // namespace bsz

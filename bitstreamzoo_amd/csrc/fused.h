@@ -12,13 +12,57 @@ struct FusedOptions {
     int chunk_rows = 16;        // 64-word rows per wave-chunk
     int warmup_rows = 1;        // speculative warm-up rows before a chunk
     int waves_per_block = 4;    // independent wavefronts per workgroup
+    int rows_per_step = 4;      // rows whose loads are issued together
     int max_ops = 4000;         // fusion depth: DAG nodes per kernel
     int max_live = 96;          // live 64-bit values per lane per kernel
     std::string cache_dir;      // code-object cache ("" = $BSZ_CACHE_DIR or ~/.cache/bsz_hip)
     bool verbose = false;
     static FusedOptions from_env();
+    // Env defaults overridden by the keys of a flat JSON object
+    // ({"chunk_rows":16,"warmup_rows":1,...}); "" leaves the defaults.
+    static FusedOptions from_json(const std::string& json);
 };
 
 std::unique_ptr<Engine> make_fused_engine(hipStream_t s, const FusedOptions& opt);
+
+// A straight-line program recorded once into a fused plan and executed over
+// any number of instances (each its own bitlength and buffers) in one launch
+// per kernel: the tiles of every instance form one device-side work queue.
+class FusedPlan {
+public:
+    // Walks `prog` with `shape`'s params (inputs need only be present, any
+    // length).  Fails if the program needs host decisions (if/while, an int
+    // expression reading a popcount).  compile=true runs hipRTC now (no GPU
+    // needed); the module is loaded on the device by set_instances().
+    static std::unique_ptr<FusedPlan> record(hipStream_t s, const Program& prog, const CaseData& shape,
+                                             const FusedOptions& opt, bool compile);
+    ~FusedPlan();
+    void set_instances(const std::vector<int64_t>& bitlengths);
+    int n_instances() const;
+    // `dev` is a row-padded stream buffer of the instance's bitlength.
+    void bind_input(int inst, const std::string& name, int64_t idx, const uint64_t* dev);
+    uint64_t* output(int inst, const std::string& name, int64_t idx);
+    // 1: fused kernels (default); 0: one kernel per DSL op over the same
+    // recorded DAG, with the same resident inputs (the plain comparator).
+    void set_executor(int executor);
+    void prepare_now();  // upload binding tables (host sync) so run() only launches
+    void run();        // asynchronous on the plan's stream
+    void capture();    // record run() into a hipGraph
+    void replay();     // launch the captured graph
+    int64_t read_int(int inst, const std::string& name, bool refresh);
+    uint32_t mismatches(int kernel);
+    std::string info_json() const;
+    const std::string& module_source() const;
+    int64_t op_count() const;
+    int n_kernels() const;
+
+private:
+    FusedPlan();
+    void load();
+    void prepare();
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+    void* graph_ = nullptr;  // per-op executor's captured graph
+};
 
 }  // namespace bsz

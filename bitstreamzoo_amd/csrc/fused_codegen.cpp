@@ -1,0 +1,543 @@
+// fused_codegen.cpp — dead-code elimination, partitioning, intermediate
+// buffer planning and HIP source generation for fused kernels (see
+// fused_graph.h for the execution model).
+#include "fused_graph.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <map>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+
+namespace bsz {
+
+namespace {
+
+bool is_free(NOp op) { return op == NOp::Input || op == NOp::Zero || op == NOp::Ones; }
+int arity(NOp op) {
+    switch (op) {
+        case NOp::Not: case NOp::Shl: case NOp::Shr: return 1;
+        case NOp::And: case NOp::Or: case NOp::Xor: case NOp::Add: return 2;
+        default: return 0;
+    }
+}
+// An advance that needs the previous row's word of an in-kernel value.
+bool stateful_shift(const GNode& n) { return n.op == NOp::Shl && n.k >= 1 && n.k <= 63; }
+// Ops that may not take an operand computed in the same kernel.
+bool needs_external_operand(const GNode& n) {
+    return n.op == NOp::Shr ? n.k != 0 : (n.op == NOp::Shl && n.k >= 64);
+}
+
+uint64_t fnv1a(const std::string& s) {
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char c : s) { h ^= c; h *= 1099511628211ull; }
+    return h;
+}
+
+// The argument block and helpers every generated module starts with.
+const char* kKernelPrelude = R"HIP(
+typedef unsigned long long u64;
+typedef unsigned int u32;
+struct InstDesc { u64 rows; u64 lw; u64 lastmask; u32 chunk_begin; u32 n_chunks; };
+struct KArgs {
+  const u64* const* in; u64* const* out; u64* pop_acc; u64* pop_out;
+  const InstDesc* inst; const u32* chunk_inst;
+  u64* st_u; u64* st_t; u32* st_flag; u32* partial; u32* mismatch; u32* ctl;
+  u32 n_chunks; u32 n_inst;
+};
+#define AGENT __HIP_MEMORY_SCOPE_AGENT
+__device__ __forceinline__ u32 lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+__device__ __forceinline__ u64 ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ u32 uni(u32 x) { return (u32)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ u64 lane63(u64 x) {
+  u32 lo = (u32)__builtin_amdgcn_readlane((int)(u32)x, 63);
+  u32 hi = (u32)__builtin_amdgcn_readlane((int)(u32)(x >> 32), 63);
+  return ((u64)hi << 32) | lo;
+}
+// x from lane-1; lane 0 receives `low` (the previous row's lane-63 word).
+__device__ __forceinline__ u64 prev_word(u64 x, u64 low) {
+  u32 lo = (u32)__builtin_amdgcn_update_dpp((int)(u32)low, (int)(u32)x, 0x138, 0xf, 0xf, false);
+  u32 hi = (u32)__builtin_amdgcn_update_dpp((int)(u32)(low >> 32), (int)(u32)(x >> 32), 0x138, 0xf, 0xf, false);
+  return ((u64)hi << 32) | lo;
+}
+__device__ __forceinline__ u64 word_mask(long long w, u64 lw, u64 lastmask) {
+  return (w < 0) ? 0ull : ((u64)w + 1 < lw ? ~0ull : ((u64)w + 1 == lw ? lastmask : 0ull));
+}
+__device__ __forceinline__ u64 ld(const u64* p, long long w, u64 n) {
+  return (w >= 0 && (u64)w < n) ? p[w] : 0ull;
+}
+__device__ __forceinline__ u64 fsl(u64 cur, u64 lower, u32 b) { return b ? ((cur << b) | (lower >> (64 - b))) : cur; }
+__device__ __forceinline__ u64 fsr(u64 cur, u64 upper, u32 b) { return b ? ((cur >> b) | (upper << (64 - b))) : cur; }
+__device__ __forceinline__ u32 wave_sum(u32 v) {
+  const u32 l = lane_id();
+  for (u32 m = 32; m > 0; m >>= 1) v += (u32)__builtin_amdgcn_ds_bpermute((int)(((l ^ m) & 63u) << 2), (int)v);
+  return v;
+}
+__device__ __forceinline__ u32 aload32(u32* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT); }
+__device__ __forceinline__ u64 aload64(u64* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT); }
+__device__ __forceinline__ void astore32(u32* p, u32 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT); }
+__device__ __forceinline__ void astore64(u64* p, u64 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT); }
+__device__ __forceinline__ void fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent"); }
+)HIP";
+
+struct Gen {
+    const SegGraph& g;
+    const PlanOptions& o;
+    std::ostringstream s;
+    Gen(const SegGraph& gg, const PlanOptions& oo) : g(gg), o(oo) {}
+};
+
+std::string hex64(uint64_t v) {
+    char b[32];
+    std::snprintf(b, sizeof b, "0x%llxull", (unsigned long long)v);
+    return b;
+}
+
+// Generates one kernel; fills ks.source and ks.n_state.
+void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
+    std::set<int32_t> in_kernel(ks.nodes.begin(), ks.nodes.end());
+    std::map<int32_t, int> input_index;     // graph node -> kernel input slot
+    for (size_t j = 0; j < ks.input_node.size(); ++j) input_index[ks.input_node[j]] = (int)j;
+
+    // state assignment
+    std::map<int32_t, int> state_of;        // node -> state word
+    int n_state = 0;
+    for (int32_t n : ks.nodes) {
+        const GNode& x = g.nodes[n];
+        if (x.op == NOp::Add || (stateful_shift(x) && in_kernel.count(x.a))) state_of[n] = n_state++;
+    }
+    ks.n_state = n_state;
+    const bool has_state = n_state > 0;
+    const int U = std::max(1, opt.rows_per_step);
+    const int CR = std::max(U, (opt.chunk_rows / U) * U);
+    const int WR = has_state ? std::max(1, opt.warmup_rows) : 0;
+
+    // which graph nodes need a plain value of an input (loaded per row)
+    std::set<int32_t> plain_inputs;
+    std::vector<int32_t> ext_shifts;        // shift nodes whose operand is not in the kernel
+    auto note_operand = [&](int32_t a) {
+        if (a >= 0 && !in_kernel.count(a) && g.nodes[a].op != NOp::Zero && g.nodes[a].op != NOp::Ones)
+            plain_inputs.insert(a);
+    };
+    for (int32_t n : ks.nodes) {
+        const GNode& x = g.nodes[n];
+        if ((x.op == NOp::Shl || x.op == NOp::Shr) && !in_kernel.count(x.a)) {
+            if (x.k == 0) note_operand(x.a);
+            else ext_shifts.push_back(n);
+            continue;
+        }
+        if (arity(x.op) >= 1) note_operand(x.a);
+        if (arity(x.op) >= 2) note_operand(x.b);
+    }
+    for (int32_t p : ks.pops) note_operand(g.pops[p]);
+    for (int32_t n : ks.outputs)
+        if (!in_kernel.count(n)) note_operand(n);
+
+    std::ostringstream s;
+    const int NIN = (int)ks.inputs.size(), NOUT = (int)ks.outputs.size(), NPOP = (int)ks.pops.size();
+    s << "extern \"C\" __global__ __launch_bounds__(" << 64 * opt.waves_per_block << ") void " << ks.name
+      << "(KArgs A) {\n";
+    s << "  const u32 lane = lane_id();\n";
+    s << "  const u32 tag = aload32(&A.ctl[2]) + 1u;\n";
+    s << "  u32 c = 0;\n  if (lane == 0) c = __hip_atomic_fetch_add(&A.ctl[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
+    s << "  c = uni(c);\n";
+    s << "  bool fix = false;\n  u32 walk = 1;\n  bool prev_changed = false;\n";
+    s << "  for (;;) {\n";
+    s << "    if (!fix && c >= A.n_chunks) return;\n";
+    s << "    const u32 ii = A.chunk_inst ? A.chunk_inst[c] : 0u;\n";
+    s << "    const InstDesc D = A.inst[ii];\n";
+    s << "    const u32 cc = c - D.chunk_begin;\n";
+    s << "    const u64 NW = D.rows * 64ull;\n";
+    for (int j = 0; j < NIN; ++j) s << "    const u64* __restrict__ in" << j << " = A.in[(u64)ii * " << NIN << "u + " << j << "u];\n";
+    for (int j = 0; j < NOUT; ++j) s << "    u64* __restrict__ out" << j << " = A.out[(u64)ii * " << NOUT << "u + " << j << "u];\n";
+    s << "    const u64 r_begin = (u64)cc * " << CR << "ull;\n";
+    s << "    const u64 r_end = (r_begin + " << CR << "ull < D.rows) ? r_begin + " << CR << "ull : D.rows;\n";
+    if (has_state) {
+        s << "    const u64 warm = (!fix && cc > 0) ? (r_begin < " << WR << "ull ? r_begin : " << WR << "ull) : 0ull;\n";
+        for (auto& kv : state_of)
+            s << "    u64 st" << kv.second << " = fix ? aload64(&A.st_t[(u64)(c - 1) * " << n_state << "ull + "
+              << kv.second << "ull]) : 0ull;\n";
+    } else {
+        s << "    const u64 warm = 0ull;\n";
+    }
+    for (int p = 0; p < NPOP; ++p) s << "    u32 pc" << p << " = 0u;\n";
+    s << "    for (u64 r = r_begin - warm; r < r_end; r += " << U << "ull) {\n";
+    // load phase
+    for (int u = 0; u < U; ++u) {
+        s << "      const long long w" << u << " = (long long)((r + " << u << "ull) * 64ull + lane);\n";
+        s << "      const bool ok" << u << " = r + " << u << "ull < r_end;\n";
+        for (int32_t a : plain_inputs)
+            s << "      const u64 x" << a << "_" << u << " = ok" << u << " ? in" << input_index.at(a) << "[w" << u << "] : 0ull;\n";
+        for (int32_t n : ext_shifts) {
+            const GNode& x = g.nodes[n];
+            const uint64_t q = x.k / 64;
+            const unsigned b = (unsigned)(x.k % 64);
+            const GNode& src = g.nodes[x.a];
+            auto load = [&](const std::string& off) -> std::string {
+                if (src.op == NOp::Zero) return "0ull";
+                if (src.op == NOp::Ones) return "word_mask(w" + std::to_string(u) + off + ", D.lw, D.lastmask)";
+                return "ld(in" + std::to_string(input_index.at(x.a)) + ", w" + std::to_string(u) + off + ", NW)";
+            };
+            std::string qs = std::to_string(q), q1 = std::to_string(q + 1);
+            std::string e = x.op == NOp::Shl
+                ? "fsl(" + load(" - " + qs + "ll") + ", " + load(" - " + q1 + "ll") + ", " + std::to_string(b) + "u)"
+                : "fsr(" + load(" + " + qs + "ll") + ", " + load(" + " + q1 + "ll") + ", " + std::to_string(b) + "u)";
+            s << "      const u64 y" << n << "_" << u << " = ok" << u << " ? " << e << " : 0ull;\n";
+        }
+    }
+    // compute phase
+    for (int u = 0; u < U; ++u) {
+        s << "      if (ok" << u << ") {\n";
+        if (has_state) {
+            // the speculative state this chunk reached at its first row (U[c])
+            s << "        if (r + " << u << "ull == r_begin && !fix && lane == 0) {\n";
+            for (auto& kv : state_of)
+                s << "          astore64(&A.st_u[(u64)c * " << n_state << "ull + " << kv.second << "ull], st" << kv.second << ");\n";
+            s << "        }\n";
+        }
+        s << "        const u64 m = word_mask(w" << u << ", D.lw, D.lastmask);\n";
+        auto ref = [&](int32_t a) -> std::string {
+            const GNode& x = g.nodes[a];
+            if (in_kernel.count(a)) return "v" + std::to_string(a);
+            if (x.op == NOp::Zero) return "0ull";
+            if (x.op == NOp::Ones) return "m";
+            return "x" + std::to_string(a) + "_" + std::to_string(u);
+        };
+        for (int32_t n : ks.nodes) {
+            const GNode& x = g.nodes[n];
+            const std::string v = "v" + std::to_string(n);
+            switch (x.op) {
+                case NOp::Zero: s << "        const u64 " << v << " = 0ull;\n"; break;
+                case NOp::Ones: s << "        const u64 " << v << " = m;\n"; break;
+                case NOp::Not: s << "        const u64 " << v << " = ~" << ref(x.a) << ";\n"; break;
+                case NOp::And: s << "        const u64 " << v << " = " << ref(x.a) << " & " << ref(x.b) << ";\n"; break;
+                case NOp::Or: s << "        const u64 " << v << " = " << ref(x.a) << " | " << ref(x.b) << ";\n"; break;
+                case NOp::Xor: s << "        const u64 " << v << " = " << ref(x.a) << " ^ " << ref(x.b) << ";\n"; break;
+                case NOp::Add: {
+                    const std::string st = "st" + std::to_string(state_of.at(n)), t = "_" + std::to_string(n);
+                    s << "        const u64 a" << t << " = " << ref(x.a) << ", s" << t << " = a" << t << " + " << ref(x.b) << ";\n";
+                    s << "        const u64 G" << t << " = ballot(s" << t << " < a" << t << ");\n";
+                    s << "        const u64 P" << t << " = G" << t << " | ballot(s" << t << " == ~0ull);\n";
+                    s << "        const u64 t" << t << " = G" << t << " + P" << t << ";\n";
+                    s << "        const u64 u" << t << " = t" << t << " + " << st << ";\n";
+                    s << "        const u64 " << v << " = s" << t << " + (((u" << t << " ^ G" << t << " ^ P" << t
+                      << ") >> lane) & 1ull);\n";
+                    s << "        " << st << " = (u64)((t" << t << " < G" << t << ") | (u" << t << " < t" << t << "));\n";
+                    break;
+                }
+                case NOp::Shl: case NOp::Shr: {
+                    if (!in_kernel.count(x.a)) {
+                        if (x.k == 0) s << "        const u64 " << v << " = " << ref(x.a) << ";\n";
+                        else s << "        const u64 " << v << " = y" << n << "_" << u << ";\n";
+                    } else if (x.k == 0) {
+                        s << "        const u64 " << v << " = " << ref(x.a) << ";\n";
+                    } else {
+                        const std::string st = "st" + std::to_string(state_of.at(n));
+                        s << "        const u64 " << v << " = fsl(" << ref(x.a) << ", prev_word(" << ref(x.a) << ", " << st
+                          << "), " << x.k << "u);\n";
+                        s << "        " << st << " = lane63(" << ref(x.a) << ");\n";
+                    }
+                    break;
+                }
+                case NOp::Input: break;
+            }
+        }
+        const bool guard = has_state;
+        if (guard) s << "        if (r + " << u << "ull >= r_begin) {\n";
+        for (int j = 0; j < NOUT; ++j)
+            s << "          out" << j << "[w" << u << "] = " << ref(ks.outputs[j]) << " & m;\n";
+        for (int p = 0; p < NPOP; ++p)
+            s << "          pc" << p << " += (u32)__builtin_popcountll(" << ref(g.pops[ks.pops[p]]) << " & m);\n";
+        if (guard) s << "        }\n";
+        s << "      }\n";
+    }
+    s << "    }\n";  // row loop
+
+    // popcount reduction and accumulation
+    for (int p = 0; p < NPOP; ++p) {
+        s << "    {\n      const u32 tot = uni(wave_sum(pc" << p << "));\n";
+        s << "      if (lane == 0) {\n";
+        s << "        u64* acc = &A.pop_acc[(u64)ii * " << NPOP << "u + " << p << "u];\n";
+        if (has_state) {
+            s << "        u32* part = &A.partial[(u64)c * " << NPOP << "u + " << p << "u];\n";
+            s << "        const u32 old = fix ? aload32(part) : 0u;\n";
+            s << "        __hip_atomic_fetch_add(acc, (u64)tot - (u64)old, __ATOMIC_RELAXED, AGENT);\n";
+            s << "        astore32(part, tot);\n";
+        } else {
+            s << "        __hip_atomic_fetch_add(acc, (u64)tot, __ATOMIC_RELAXED, AGENT);\n";
+        }
+        s << "      }\n    }\n";
+    }
+
+    if (has_state) {
+        // publish T[c]; in fix mode also learn whether it changed
+        s << "    bool changed = false;\n";
+        s << "    {\n";
+        for (auto& kv : state_of) {
+            s << "      if (lane == " << (kv.second % 64) << ") {\n";
+            s << "        u64* tp = &A.st_t[(u64)c * " << n_state << "ull + " << kv.second << "ull];\n";
+            s << "        if (fix && aload64(tp) != st" << kv.second << ") changed = true;\n";
+            s << "        astore64(tp, st" << kv.second << ");\n";
+            s << "      }\n";
+        }
+        s << "      changed = ballot(changed) != 0ull;\n";
+        s << "      fence();\n";
+        s << "      if (lane == 0) astore32(&A.st_flag[c], tag);\n";
+        s << "    }\n";
+        // main mode: verify against the predecessor's published end state
+        s << "    if (!fix) {\n";
+        s << "      bool bad = false;\n";
+        s << "      if (cc > 0) {\n";
+        s << "        if (lane == 0) { while (aload32(&A.st_flag[c - 1]) != tag) __builtin_amdgcn_s_sleep(1); }\n";
+        s << "        fence();\n";
+        s << "        for (u32 i = lane; i < " << n_state << "u; i += 64u)\n";
+        s << "          if (aload64(&A.st_u[(u64)c * " << n_state << "ull + i]) != aload64(&A.st_t[(u64)(c - 1) * "
+          << n_state << "ull + i])) bad = true;\n";
+        s << "        bad = ballot(bad) != 0ull;\n";
+        s << "      }\n";
+        s << "      if (lane == 0) {\n        astore32(&A.mismatch[c], bad ? 1u : 0u);\n";
+        s << "        if (bad) __hip_atomic_fetch_add(&A.ctl[3], 1u, __ATOMIC_RELAXED, AGENT);\n      }\n";
+        s << "    }\n";
+    }
+
+    // next job
+    s << "    if (!fix) {\n";
+    s << "      fence();\n";
+    s << "      u32 d = 0;\n      if (lane == 0) d = __hip_atomic_fetch_add(&A.ctl[1], 1u, __ATOMIC_RELAXED, AGENT);\n";
+    s << "      d = uni(d);\n";
+    s << "      if (d != A.n_chunks - 1u) {\n";
+    s << "        if (lane == 0) c = __hip_atomic_fetch_add(&A.ctl[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
+    s << "        c = uni(c);\n        continue;\n      }\n";
+    s << "      fence();\n";
+    if (has_state) {
+        s << "      if (aload32(&A.ctl[3]) == 0u) break;\n";
+        s << "      fix = true;\n      walk = 1u;\n      prev_changed = false;\n";
+    } else {
+        s << "      break;\n";
+    }
+    if (has_state) {
+        s << "    } else {\n";
+        s << "      prev_changed = changed;\n      walk = c + 1u;\n";
+    }
+    s << "    }\n";
+    if (has_state) {
+        // find the next chunk to recompute, in order
+        s << "    {\n      u32 nxt = 0xffffffffu;\n";
+        s << "      while (walk < A.n_chunks) {\n";
+        s << "        if (prev_changed) {\n";
+        s << "          const u32 pi = A.chunk_inst ? A.chunk_inst[walk] : 0u;\n";
+        s << "          bool need = false;\n";
+        s << "          if (walk != A.inst[pi].chunk_begin) {\n";
+        s << "            for (u32 i = lane; i < " << n_state << "u; i += 64u)\n";
+        s << "              if (aload64(&A.st_u[(u64)walk * " << n_state << "ull + i]) != aload64(&A.st_t[(u64)(walk - 1) * "
+          << n_state << "ull + i])) need = true;\n";
+        s << "            need = ballot(need) != 0ull;\n";
+        s << "          }\n";
+        s << "          if (need) { nxt = walk; break; }\n";
+        s << "          prev_changed = false;\n          ++walk;\n          continue;\n";
+        s << "        }\n";
+        s << "        const u32 x = walk + lane;\n";
+        s << "        const u64 hits = ballot(x < A.n_chunks && aload32(&A.mismatch[x]) != 0u);\n";
+        s << "        if (hits) { nxt = walk + (u32)__builtin_ctzll(hits); break; }\n";
+        s << "        walk += 64u;\n";
+        s << "      }\n";
+        s << "      if (nxt == 0xffffffffu) break;\n";
+        s << "      c = nxt;\n";
+        s << "    }\n";
+    }
+    s << "  }\n";  // job loop
+    s << "}\n";
+    // The control block and the popcount accumulators are reset by the plan's
+    // prep kernel before every launch, so a wave that starts after the last
+    // chunk finished only finds the ticket counter past n_chunks.
+    ks.source = s.str();
+}
+
+}  // namespace
+
+std::string SegGraph::key() const {
+    std::ostringstream s;
+    s << "S" << n_sources << ";";
+    for (const GNode& n : nodes) s << (int)n.op << "," << n.a << "," << n.b << "," << n.k << "," << n.src << ";";
+    s << "|O";
+    for (int32_t o : outputs) s << o << ",";
+    s << "|P";
+    for (int32_t p : pops) s << p << ",";
+    return s.str();
+}
+
+std::string PlanOptions::key() const {
+    std::ostringstream s;
+    s << chunk_rows << "," << warmup_rows << "," << waves_per_block << "," << rows_per_step << "," << max_ops << ","
+      << max_live;
+    return s.str();
+}
+
+PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt) {
+    PlanSpec P;
+    P.g = g;
+    P.opt = opt;
+    const int32_t N = (int32_t)g.nodes.size();
+
+    // ── liveness / dead-code elimination ──
+    std::vector<char> needed(N, 0), is_out(N, 0);
+    for (int32_t o : g.outputs) { needed[o] = 1; is_out[o] = 1; }
+    for (int32_t p : g.pops) needed[p] = 1;
+    for (int32_t n = N - 1; n >= 0; --n) {
+        if (!needed[n]) continue;
+        const GNode& x = g.nodes[n];
+        if (arity(x.op) >= 1) needed[x.a] = 1;
+        if (arity(x.op) >= 2) needed[x.b] = 1;
+    }
+    std::vector<int32_t> last_use(N, -1);
+    for (int32_t n = 0; n < N; ++n) {
+        if (!needed[n]) continue;
+        const GNode& x = g.nodes[n];
+        if (arity(x.op) >= 1) last_use[x.a] = std::max(last_use[x.a], n);
+        if (arity(x.op) >= 2) last_use[x.b] = std::max(last_use[x.b], n);
+    }
+    for (int32_t o : g.outputs) last_use[o] = N;
+    for (int32_t p : g.pops) last_use[p] = N;
+
+    // ── greedy linear partitioning ──
+    std::vector<int32_t> kern_of(N, -1);
+    std::vector<std::vector<int32_t>> kernels;
+    int live = 0, n_state = 0;
+    for (int32_t n = 0; n < N; ++n) {
+        if (!needed[n]) continue;
+        const GNode& x = g.nodes[n];
+        if (is_free(x.op)) continue;
+        const int K = (int)kernels.size() - 1;
+        bool cut = kernels.empty();
+        if (!cut) {
+            auto inside = [&](int32_t a) { return a >= 0 && kern_of[a] == K; };
+            if (needs_external_operand(x) && inside(x.a)) cut = true;
+            if ((int)kernels.back().size() >= opt.max_ops) cut = true;
+            int delta = last_use[n] > n ? 1 : 0;
+            if (arity(x.op) >= 1 && inside(x.a) && last_use[x.a] == n) --delta;
+            if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && last_use[x.b] == n) --delta;
+            if (live + delta > opt.max_live) cut = true;
+            if (n_state >= 512 && (x.op == NOp::Add || (stateful_shift(x) && inside(x.a)))) cut = true;
+        }
+        if (cut) {
+            kernels.emplace_back();
+            live = 0;
+            n_state = 0;
+        }
+        const int KK = (int)kernels.size() - 1;
+        auto inside = [&](int32_t a) { return a >= 0 && kern_of[a] == KK; };
+        if (arity(x.op) >= 1 && inside(x.a) && last_use[x.a] == n) --live;
+        if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && last_use[x.b] == n) --live;
+        if (last_use[n] > n) ++live;
+        if (x.op == NOp::Add || (stateful_shift(x) && inside(x.a))) ++n_state;
+        kern_of[n] = KK;
+        kernels.back().push_back(n);
+    }
+    // popcounts and outputs of free nodes (inputs / constants) ride on the last kernel
+    bool free_work = false;
+    for (int32_t p : g.pops) if (kern_of[p] < 0) free_work = true;
+    for (int32_t o : g.outputs) if (kern_of[o] < 0 && g.nodes[o].op != NOp::Input) free_work = true;
+    if (kernels.empty() && free_work) kernels.emplace_back();
+
+    const int NK = (int)kernels.size();
+    P.kernels.resize(NK);
+    for (int k = 0; k < NK; ++k) {
+        char nm[64];
+        std::snprintf(nm, sizeof nm, "bsz_k%d", k);
+        P.kernels[k].name = nm;
+        P.kernels[k].nodes = kernels[k];
+    }
+
+    // consumers per kernel, for materialization and slot intervals
+    std::vector<int> last_kernel_use(N, -1);
+    auto use_in = [&](int32_t a, int k) {
+        if (a >= 0 && kern_of[a] != k) last_kernel_use[a] = std::max(last_kernel_use[a], k);
+    };
+    for (int k = 0; k < NK; ++k) {
+        for (int32_t n : kernels[k]) {
+            const GNode& x = g.nodes[n];
+            if (arity(x.op) >= 1) use_in(x.a, k);
+            if (arity(x.op) >= 2) use_in(x.b, k);
+        }
+    }
+    std::vector<int> pop_kernel(g.pops.size());
+    for (size_t p = 0; p < g.pops.size(); ++p) {
+        int32_t n = g.pops[p];
+        int k = kern_of[n] >= 0 ? kern_of[n] : NK - 1;
+        pop_kernel[p] = k;
+        P.kernels[k].pops.push_back((int32_t)p);
+        use_in(n, k);
+    }
+    // outputs of free nodes (constants) are stored by the last kernel
+    std::vector<int> out_kernel(N, -1);
+    for (int32_t o : g.outputs) {
+        if (kern_of[o] >= 0) out_kernel[o] = kern_of[o];
+        else if (g.nodes[o].op != NOp::Input) { out_kernel[o] = NK - 1; use_in(o, NK - 1); }
+    }
+
+    // ── materialized intermediates and slot reuse (interval colouring) ──
+    std::vector<int32_t> slot_of(N, -1);
+    std::vector<std::pair<int, int32_t>> free_slots;  // (free after kernel, slot)
+    std::vector<int> slot_busy_until;
+    int n_slots = 0;
+    for (int k = 0; k < NK; ++k) {
+        for (int32_t n : kernels[k]) {
+            if (is_out[n] || last_kernel_use[n] <= k) continue;  // segment outputs own their buffers
+            int32_t slot = -1;
+            for (int s = 0; s < n_slots; ++s)
+                if (slot_busy_until[s] < k) { slot = s; break; }
+            if (slot < 0) { slot = n_slots++; slot_busy_until.push_back(0); }
+            slot_busy_until[slot] = last_kernel_use[n];
+            slot_of[n] = slot;
+        }
+    }
+    P.n_slots = n_slots;
+
+    // ── kernel interfaces ──
+    for (int k = 0; k < NK; ++k) {
+        KernelSpec& ks = P.kernels[k];
+        std::set<int32_t> ins;
+        auto ext = [&](int32_t a) {
+            if (a < 0 || kern_of[a] == k) return;
+            if (g.nodes[a].op == NOp::Zero || g.nodes[a].op == NOp::Ones) return;
+            ins.insert(a);
+        };
+        for (int32_t n : ks.nodes) {
+            const GNode& x = g.nodes[n];
+            if (arity(x.op) >= 1) ext(x.a);
+            if (arity(x.op) >= 2) ext(x.b);
+        }
+        for (int32_t p : ks.pops) ext(g.pops[p]);
+        for (int32_t a : ins) {
+            KInput ki;
+            const GNode& x = g.nodes[a];
+            if (x.op == NOp::Input) { ki.kind = KInput::Source; ki.id = x.src; }
+            else if (is_out[a]) { ki.kind = KInput::SegOut; ki.id = a; }
+            else { ki.kind = KInput::Slot; ki.id = slot_of[a]; }
+            ks.inputs.push_back(ki);
+            ks.input_node.push_back(a);
+        }
+        for (int32_t n : ks.nodes) {
+            if (is_out[n]) { ks.outputs.push_back(n); ks.out_slot.push_back(-1); }
+            else if (slot_of[n] >= 0) { ks.outputs.push_back(n); ks.out_slot.push_back(slot_of[n]); }
+        }
+        for (int32_t o : g.outputs)
+            if (out_kernel[o] == k && kern_of[o] < 0) { ks.outputs.push_back(o); ks.out_slot.push_back(-1); }
+    }
+    P.out_node_kernel.clear();
+    for (int32_t o : g.outputs) P.out_node_kernel.push_back(out_kernel[o]);
+
+    std::ostringstream mod;
+    mod << kKernelPrelude;
+    for (KernelSpec& ks : P.kernels) {
+        gen_kernel(g, opt, ks);
+        mod << ks.source;
+    }
+    P.module_source = mod.str();
+    return P;
+}
+
+uint64_t source_hash(const std::string& s) { return fnv1a(s); }
+
+}  // namespace bsz

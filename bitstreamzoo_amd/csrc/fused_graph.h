@@ -1,0 +1,119 @@
+// fused_graph.h — segment DAG, fusion partitioner, stream memory planner and
+// kernel code generation for the fused executor.
+//
+// Execution model of a fused kernel (one per partition):
+//   * a stream is cut into rows of 64 words; one wavefront owns a chunk of
+//     `chunk_rows` consecutive rows and walks them in order, one 64-bit word
+//     per lane per row, every live value in registers;
+//   * an advance by k < 64 of a value computed in the same kernel takes the
+//     lower neighbour word with a DPP wave_shr:1 lane move, lane 0 taking the
+//     previous row's lane 63 word, kept in a scalar register;
+//   * stream `+` forms the 64 lane sums, ballots their generate / propagate
+//     predicates and resolves the row's carries with scalar arithmetic on the
+//     two masks (C = (G + (G|P) + cin) ^ G ^ (G|P)); the carry out of the row
+//     is the next row's carry in;
+//   * those scalars (shift neighbours and carries) are the kernel's state.  A
+//     chunk other than the first starts `warmup_rows` rows early with a zero
+//     state (speculation), records the state it reaches at its first row (U)
+//     and publishes the state at its end (T).  Chunk c was exact iff
+//     U[c] == T[c-1]; each chunk checks that against its predecessor, and the
+//     last wave of the launch recomputes, in order, every chunk whose check
+//     failed from its predecessor's true end state.  The result is exact for
+//     every input; speculation only decides how much work is redone;
+//   * an advance of a value computed by an earlier kernel (any k) and `>>` are
+//     direct loads at the shifted word offset, needing no state;
+//   * popcounts accumulate per lane (v_bcnt), are reduced per wave and added
+//     atomically per chunk.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bsz {
+
+enum class NOp : uint8_t { Input, Zero, Ones, Not, And, Or, Xor, Add, Shl, Shr };
+
+struct GNode {
+    NOp op;
+    int32_t a = -1, b = -1;
+    uint64_t k = 0;      // shift amount
+    int32_t src = -1;    // Input: index of the external source
+};
+
+struct SegGraph {
+    std::vector<GNode> nodes;        // topological (creation) order
+    std::vector<int32_t> outputs;    // nodes to materialize
+    std::vector<int32_t> pops;       // nodes to popcount
+    int32_t n_sources = 0;
+    std::string key() const;         // structural identity (no buffers, no sizes)
+};
+
+struct PlanOptions {
+    int chunk_rows = 16;
+    int warmup_rows = 1;
+    int waves_per_block = 4;
+    int rows_per_step = 4;           // rows whose loads are issued together
+    int max_ops = 4000;
+    int max_live = 96;
+    std::string key() const;
+};
+
+// Where a kernel reads an input from.
+struct KInput {
+    enum Kind : uint8_t { Source, Slot, SegOut } kind = Source;
+    int32_t id = -1;                 // source index, intermediate slot, or graph node of a segment output
+};
+
+struct KernelSpec {
+    std::string name;
+    std::vector<int32_t> nodes;      // computed nodes, in order
+    std::vector<KInput> inputs;
+    std::vector<int32_t> input_node; // graph node id of each input (Input node or producer)
+    std::vector<int32_t> outputs;    // nodes this kernel stores
+    std::vector<int32_t> out_slot;   // per output: -1 segment output, else intermediate slot
+    std::vector<int32_t> pops;       // index into SegGraph::pops accumulated here
+    int n_state = 0;                 // 64-bit state words
+    std::string source;              // generated HIP
+};
+
+struct PlanSpec {
+    SegGraph g;
+    PlanOptions opt;
+    std::vector<KernelSpec> kernels;
+    int32_t n_slots = 0;             // intermediate buffers (after interval reuse)
+    std::vector<int32_t> out_node_kernel;  // per g.outputs entry: kernel storing it, -1 if it is a source
+    std::string module_source;       // all kernels, one hipRTC program
+};
+
+// Dead-code elimination, partitioning, slot assignment and code generation.
+PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt);
+
+// Argument block every generated kernel takes by value (layout shared with the
+// generated source, see kKernelPrelude in fused_codegen.cpp).
+struct InstDesc {
+    uint64_t rows;
+    uint64_t lw;
+    uint64_t lastmask;
+    uint32_t chunk_begin;
+    uint32_t n_chunks;
+};
+
+struct KArgs {
+    const uint64_t* const* in;        // [inst][n_in]
+    uint64_t* const* out;             // [inst][n_out]
+    unsigned long long* pop_acc;      // [inst][n_pop_k]   zeroed again by the last wave
+    unsigned long long* pop_out;      // [inst][n_pop_k]
+    const InstDesc* inst;
+    const uint32_t* chunk_inst;       // [n_chunks] instance of each chunk
+    unsigned long long* st_u;         // [n_chunks][n_state]
+    unsigned long long* st_t;         // [n_chunks][n_state]
+    unsigned* st_flag;                // [n_chunks] epoch tag of a published T
+    unsigned* partial;                // [n_chunks][n_pop_k]
+    unsigned* mismatch;               // [n_chunks]
+    unsigned* ctl;                    // ticket, done, epoch, mismatches, recomputed
+    uint32_t n_chunks;
+    uint32_t n_inst;
+};
+
+}  // namespace bsz

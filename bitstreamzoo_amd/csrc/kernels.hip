@@ -246,6 +246,17 @@ __global__ __launch_bounds__(256) void k_tail(uint64_t* dst, uint64_t k, uint64_
     if (blockIdx.x == 0 && threadIdx.x == 0 && lw >= 1 && lw - 1 < k) dst[lw - 1] &= lastmask;
 }
 
+__global__ __launch_bounds__(256) void k_plan_prep(unsigned* ctl, uint32_t n_ctl, unsigned long long* pop, uint64_t n_pop) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_ctl) {
+        ctl[4 * i + 0] = 0u;
+        ctl[4 * i + 1] = 0u;
+        ctl[4 * i + 2] += 1u;
+        ctl[4 * i + 3] = 0u;
+    }
+    if (i < n_pop) pop[i] = 0ull;
+}
+
 __global__ __launch_bounds__(256) void k_copy_words(uint64_t* __restrict__ dst, const uint64_t* __restrict__ src,
                                                     uint64_t n_pairs) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -334,6 +345,13 @@ void upload_stream(hipStream_t s, uint64_t* dst, const uint64_t* host, uint64_t 
     if (k) BSZ_HIP(hipMemcpyAsync(dst, host, k * 8, hipMemcpyHostToDevice, s));
     const uint64_t total = g.alloc_words();
     k_tail<<<std::max(1u, blocks_for(total - k, 256)), 256, 0, s>>>(dst, k, total, g.lw, g.lastmask);
+    check_launch();
+}
+
+void launch_plan_prep(hipStream_t s, unsigned* ctl, uint32_t n_ctl, unsigned long long* pop, uint64_t n_pop) {
+    const uint64_t n = std::max<uint64_t>(n_ctl, n_pop);
+    if (!n) return;
+    k_plan_prep<<<blocks_for(n, 256), 256, 0, s>>>(ctl, n_ctl, pop, n_pop);
     check_launch();
 }
 

@@ -123,6 +123,59 @@ uint64_t bsz_result_n_words(const bsz_result* r);
  * cap bytes (NUL-terminated) and returns the full length. */
 int64_t bsz_result_json(const bsz_result* r, char* buf, uint64_t cap);
 
+/* ── fused plans: a straight-line program recorded once, then run over many
+ *    instances (configuration batches of uneven lengths) in one launch per
+ *    kernel; the reference runs one process per .bsdata file and one kernel
+ *    per op (base.py:228-254, interpreter.inl:314-316), so these have no
+ *    direct counterpart there ──────────────────────────────────────────── */
+
+/* Records the program with `shape`'s params (every input it reads must be
+ * present in `shape`, any length) and compiles it.  `options_json` is "" or a
+ * flat object with chunk_rows, warmup_rows, waves_per_block, rows_per_step,
+ * max_ops, max_live.  BSZ_E_RUNTIME if the program needs host decisions
+ * (if/while, int expressions that read a popcount). */
+int bsz_plan_create(bsz_ctx* ctx, const bsz_program* p, const bsz_case* shape, const char* options_json,
+                    bsz_plan** out);
+/* Record + generate + hipRTC-compile only (no device needed); writes the plan
+ * description JSON. */
+int64_t bsz_plan_compile_only(const bsz_program* p, const bsz_case* shape, const char* options_json,
+                              char* info, uint64_t cap);
+int bsz_plan_free(bsz_plan* plan);
+/* One instance per bitlength; allocates the plan's output and scratch buffers. */
+int bsz_plan_set_instances(bsz_plan* plan, int n, const int64_t* bitlengths);
+/* Binds a device stream (from bsz_buf_alloc, instance's bitlength) as input
+ * `name` (array element `idx`, or -1) of instance `inst`. */
+int bsz_plan_bind_input(bsz_plan* plan, int inst, const char* name, int64_t idx, const uint64_t* dev);
+/* Device buffer of output `name` (array element idx or -1) of instance inst. */
+int bsz_plan_output(bsz_plan* plan, int inst, const char* name, int64_t idx, uint64_t** dev_out);
+/* Launch asynchronously on the context stream. */
+int bsz_plan_run(bsz_plan* plan);
+/* Capture one run into a hipGraph; bsz_plan_replay launches it `times` times. */
+int bsz_plan_capture(bsz_plan* plan);
+int bsz_plan_replay(bsz_plan* plan, int times);
+/* Int output (a popcount) of instance inst after the last run (synchronous). */
+int bsz_plan_read_int(bsz_plan* plan, int inst, const char* name, int64_t* out);
+/* Chunks whose speculative start state was wrong in the last run, kernel k. */
+int bsz_plan_mismatches(bsz_plan* plan, int kernel, int64_t* out);
+/* BSZ_EXEC_FUSED (default) or BSZ_EXEC_PEROP: one kernel per DSL op over the
+ * same recorded DAG and the same bound buffers (the plain comparator). */
+int bsz_plan_set_executor(bsz_plan* plan, int executor);
+/* Upload binding tables now (host sync), so that run/capture only launch. */
+int bsz_plan_prepare(bsz_plan* plan);
+int64_t bsz_plan_info(const bsz_plan* plan, char* buf, uint64_t cap);
+int64_t bsz_plan_source(const bsz_plan* plan, char* buf, uint64_t cap);
+
+/* ── device timing and graph capture on the context stream (hipEvent /
+ *    hipGraph; replaces GpuTimer, gpu_timer.cuh:6-29) ───────────────────── */
+
+int bsz_timer_start(bsz_ctx* ctx);
+/* Elapsed milliseconds between the start event and now (synchronous). */
+int bsz_timer_stop(bsz_ctx* ctx, double* ms);
+int bsz_capture_begin(bsz_ctx* ctx);
+int bsz_capture_end(bsz_ctx* ctx, void** graph_exec);
+int bsz_graph_launch(bsz_ctx* ctx, void* graph_exec, int times);
+int bsz_graph_destroy(void* graph_exec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,85 @@
+"""The fused executor's code generator on the CPU: plans are recorded,
+partitioned and compiled by hipRTC for gfx950 (no device needed) for the
+benchmark workloads, straight-line reference programs and random programs,
+under several tile configurations."""
+
+import numpy as np
+import pytest
+
+import progfuzz
+import ref_fixtures
+from bitstreamzoo_amd import api, workloads
+from bitstreamzoo_amd._lib import BszError
+from bitstreamzoo_amd.bsdata import BsCase, read_cases
+
+OPTS = [{}, {"chunk_rows": 1, "rows_per_step": 1}, {"chunk_rows": 8, "rows_per_step": 2, "warmup_rows": 2},
+        {"max_ops": 7, "max_live": 6}]
+
+
+def shape_for(src, bl=64):
+    rng = np.random.Generator(np.random.PCG64(0))
+    return api.Case.from_bscase(progfuzz.random_case(0, bl))
+
+
+@pytest.mark.parametrize("opt", OPTS, ids=[str(o) for o in OPTS])
+def test_workload_plans_compile(opt):
+    rng = np.random.Generator(np.random.PCG64(1))
+    cases = [(workloads.MATCHSTAR_BS, workloads.matchstar(256).cases[0]),
+             (workloads.adder32_source(8), workloads.adder32(256, width=8).cases[0]),
+             (workloads.shift_and_source("ACGTTGCAACGT"), workloads.dna_case(rng, 256, "s"))]
+    for src, case in cases:
+        info = api.compile_plan_only(api.Program(src), api.Case.from_bscase(case), opt)
+        assert info["kernels"], info
+        if opt.get("max_ops") == 7:
+            assert len(info["kernels"]) >= 2 or info["graph_nodes"] < 10
+
+
+def test_matchstar_plan_shape():
+    info = api.compile_plan_only(api.Program(workloads.MATCHSTAR_BS),
+                                 api.Case.from_bscase(workloads.matchstar(256).cases[0]))
+    k = info["kernels"]
+    assert len(k) == 1 and k[0]["nodes"] == 4 and k[0]["state_words"] == 1 and k[0]["pops"] == 1
+    assert info["slots"] == 0
+
+
+def test_host_decisions_are_rejected_by_plan_recording():
+    src = "input stream a\noutput stream o\nif (a) { o = ~a }\n"
+    c = BsCase(bitlength=64, inputs={"a": np.zeros(1, np.uint64)})
+    with pytest.raises(BszError) as e:
+        api.compile_plan_only(api.Program(src), api.Case.from_bscase(c))
+    assert e.value.code == -3
+
+
+def _straight_line_fuzz():
+    out = []
+    for seed in range(60):
+        src = progfuzz.random_program(seed, n_stmts=40, allow_control=False)
+        out.append((seed, src))
+    return out
+
+
+@pytest.mark.parametrize("seed,src", _straight_line_fuzz()[:24], ids=lambda x: str(x)[:6])
+def test_random_straight_line_plans_compile(seed, src):
+    case = api.Case.from_bscase(progfuzz.random_case(seed, 200))
+    try:
+        info = api.compile_plan_only(api.Program(src), case, OPTS[seed % len(OPTS)])
+    except BszError as e:
+        assert e.code == -3, str(e)[:3000]  # reads a popcount on the host: not recordable
+        return
+    assert info["graph_nodes"] > 0
+
+
+@pytest.mark.skipif(not ref_fixtures.available(), reason="reference tree not present")
+def test_reference_straight_line_programs_compile():
+    n_ok = 0
+    for cid, bs, path in ref_fixtures.unit_cases()[::4]:
+        src = open(bs).read()
+        if "if (" in src or "while" in src or len(src) > 60000:
+            continue
+        case = read_cases(path)[0]
+        try:
+            api.compile_plan_only(api.Program(src), api.Case.from_bscase(case))
+            n_ok += 1
+        except BszError as e:
+            assert e.code == -3, (cid, str(e)[:2000])
+    assert n_ok >= 5
