@@ -81,14 +81,29 @@ def time_graph(ctx, graph, steps, warmup, reps_per_launch=1):
     return ms / (steps * reps_per_launch)
 
 
-def time_plans(ctx, plans, steps, warmup, executor):
-    """Capture one graph that runs every plan once; time `steps` replays.
+MAX_GRAPH_LAUNCHES = 20000
+
+
+def time_plans(ctx, plans, steps, warmup, executor, launches_per_run=1):
+    """Time `steps` runs of every plan.  Runs are captured into one hipGraph
+    and replayed, unless a run enqueues more than MAX_GRAPH_LAUNCHES kernels
+    (the per-op executor over a large batch), which is timed eagerly.
     Returns ms per plan run."""
     for p in plans:
         p.set_executor(executor)
         p.prepare()
         p.run()
     ctx.sync()
+    if launches_per_run * len(plans) > MAX_GRAPH_LAUNCHES:
+        for _ in range(warmup):
+            for p in plans:
+                p.run()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(steps):
+            for p in plans:
+                p.run()
+        return ctx.timer_stop() / (steps * len(plans))
     g = ctx.capture(lambda: [p.run() for p in plans])
     ms = time_graph(ctx, g, steps, warmup, reps_per_launch=len(plans))
     del g
@@ -154,9 +169,11 @@ def bench_adder(ctx, args, perop=True):
     if not ok:
         raise SystemExit("adder32 outputs differ from the host computation")
     ms = time_plans(ctx, [B.plan], args.steps, args.warmup, api.FUSED)
+    log(f"adder32: fused {ms:.4f} ms/step")
     res = {"ms_per_step": ms, "bit_ops": w.bit_ops, "alg_bytes": w.alg_bytes, "plan": B.plan.info()}
     if perop:
         res["perop_ms_per_step"] = time_plans(ctx, [B.plan], max(2, args.steps // 4), 1, api.PEROP)
+        log(f"adder32: per-op {res['perop_ms_per_step']:.4f} ms/step")
     B.free()
     return w, res
 
@@ -176,7 +193,9 @@ def bench_matchstar(ctx, args):
         if not np.array_equal(got, e) or B.plan.read_int(0, "n") != words_popcount(e):
             raise SystemExit("matchstar output differs from the host computation")
     ms = time_plans(ctx, [b.plan for b in bound], args.steps, args.warmup, api.FUSED)
+    log(f"matchstar: fused {ms:.4f} ms/step")
     perop = time_plans(ctx, [b.plan for b in bound], max(2, args.steps // 4), 1, api.PEROP)
+    log(f"matchstar: per-op {perop:.4f} ms/step")
     info = bound[0].plan.info()
     for B in bound:
         B.free()
@@ -189,6 +208,7 @@ def bench_shift_and(ctx, args):
     log(f"shift_and: generating {n} sequences, lengths log-uniform in [2^12, 2^22]")
     w = workloads.shift_and_batch(n_seqs=n, seed=3)
     B = Bound(ctx, w, w.cases, {"chunk_rows": 16, "rows_per_step": 4})
+    log(f"shift_and: plan bound over {n} instances ({B.plan.info()['n_chunks']} chunks); running")
     B.plan.run()
     ctx.sync()
     pattern = w.source.split("the 32-mer ")[1].split(":")[0]
@@ -197,8 +217,12 @@ def bench_shift_and(ctx, args):
     for i in check_idx:
         if B.plan.read_int(i, "n") != shift_and_expected_popcount(w.cases[i], pattern):
             raise SystemExit(f"shift_and popcount of sequence {i} differs from the host computation")
+    log("shift_and: outputs checked; timing the fused plan")
     ms = time_plans(ctx, [B.plan], args.steps, args.warmup, api.FUSED)
-    perop = time_plans(ctx, [B.plan], 2, 1, api.PEROP)
+    log(f"shift_and: fused {ms:.4f} ms/step; timing the per-op executor (eager launches)")
+    ops_per_inst = B.plan.info()["graph_nodes"]
+    perop = time_plans(ctx, [B.plan], 2, 1, api.PEROP, launches_per_run=n * ops_per_inst)
+    log(f"shift_and: per-op {perop:.4f} ms/step")
     info = B.plan.info()
     B.free()
     return w, {"ms_per_step": ms, "perop_ms_per_step": perop, "bit_ops": w.bit_ops, "alg_bytes": w.alg_bytes,

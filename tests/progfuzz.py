@@ -15,7 +15,8 @@ from bitstreamzoo_amd.bsdata import BsCase, n_words_for
 
 
 class _Gen:
-    def __init__(self, rng: np.random.Generator, n_stmts: int, allow_control: bool, big_shifts: bool):
+    def __init__(self, rng: np.random.Generator, n_stmts: int, allow_control: bool, big_shifts: bool,
+                 host_ints: bool = True):
         self.r = rng
         self.lines: list[str] = []
         self.streams = ["x0", "x1", "x2"]
@@ -25,6 +26,7 @@ class _Gen:
         self.control = allow_control
         self.big_shifts = big_shifts
         self.depth = 0
+        self.host_ints = host_ints
 
     def fresh(self, p="v"):
         self.n += 1
@@ -105,7 +107,7 @@ class _Gen:
             self.depth -= 1
             self.emit("}")
             return
-        if r < 0.17:
+        if r < 0.17 and self.host_ints:
             v = self.fresh("n")
             self.emit(f"int {v} = popcount({self.r.choice(self.streams)})")
             if self.depth == 0:
@@ -141,8 +143,12 @@ class _Gen:
         return "\n".join(hdr + self.lines + tail) + "\n"
 
 
-def random_program(seed: int, n_stmts: int = 30, allow_control: bool = True, big_shifts: bool = True) -> str:
-    return _Gen(np.random.Generator(np.random.PCG64(seed)), n_stmts, allow_control, big_shifts).program()
+def random_program(seed: int, n_stmts: int = 30, allow_control: bool = True, big_shifts: bool = True,
+                   host_ints: bool = True) -> str:
+    """host_ints=False: popcounts only feed int outputs, never int
+    expressions, so the program records as one fused plan."""
+    return _Gen(np.random.Generator(np.random.PCG64(seed)), n_stmts, allow_control, big_shifts,
+                host_ints).program()
 
 
 def random_case(seed: int, bitlength: int, density: float | None = None) -> BsCase:
