@@ -338,10 +338,18 @@ public:
             A.ctl = (unsigned*)ptr(off_ctl_ + 16ull * k);
             A.n_chunks = n_chunks;
             A.n_inst = n_inst;
+            A.mode = 0;
             void* params[] = {&A};
             BSZ_HIP(hipModuleLaunchKernel(mod->fns[k], blocks, 1, 1, 64 * wpb, 1, 1, 0, s, params, nullptr));
+            ++launches_;
+            if (r.nst) {
+                // in-order recomputation of mis-speculated chunks (returns at once when there are none)
+                A.mode = 1;
+                BSZ_HIP(hipModuleLaunchKernel(mod->fns[k], 1, 1, 1, 64, 1, 1, 0, s, params, nullptr));
+                ++launches_;
+            }
         }
-        launches_ += 1 + (int64_t)P.kernels.size();
+        ++launches_;
     }
 
     void capture(hipStream_t s) {

@@ -44,7 +44,7 @@ struct KArgs {
   const u64* const* in; u64* const* out; u64* pop_acc; u64* pop_out;
   const InstDesc* inst; const u32* chunk_inst;
   u64* st_u; u64* st_t; u32* st_flag; u32* partial; u32* mismatch; u32* ctl;
-  u32 n_chunks; u32 n_inst;
+  u32 n_chunks; u32 n_inst; u32 mode;
 };
 #define AGENT __HIP_MEMORY_SCOPE_AGENT
 __device__ __forceinline__ u32 lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
@@ -78,7 +78,15 @@ __device__ __forceinline__ u32 aload32(u32* p) { return __hip_atomic_load(p, __A
 __device__ __forceinline__ u64 aload64(u64* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT); }
 __device__ __forceinline__ void astore32(u32* p, u32 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT); }
 __device__ __forceinline__ void astore64(u64* p, u64 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT); }
-__device__ __forceinline__ void fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent"); }
+// Wait until this wave's outstanding memory operations completed.  Cross-wave
+// data (states, flags, mismatch marks) is written and read with device-scope
+// atomics, which are coherent across XCDs without an L2 writeback, so a
+// completion wait before the flag store is all the ordering needed.
+__device__ __forceinline__ void drain() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xF70);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 )HIP";
 
 struct Gen {
@@ -140,11 +148,40 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
       << "(KArgs A) {\n";
     s << "  const u32 lane = lane_id();\n";
     s << "  const u32 tag = aload32(&A.ctl[2]) + 1u;\n";
-    s << "  u32 c = 0;\n  if (lane == 0) c = __hip_atomic_fetch_add(&A.ctl[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
-    s << "  c = uni(c);\n";
-    s << "  bool fix = false;\n  u32 walk = 1;\n  bool prev_changed = false;\n";
+    // mode 0: the main launch, a queue of chunks; mode 1: one wave that
+    // recomputes, in order, the chunks whose speculative start state was wrong
+    s << "  const bool fix = A.mode != 0u;\n";
+    s << "  u32 c = 0;\n";
+    s << "  if (!fix) {\n    if (lane == 0) c = __hip_atomic_fetch_add(&A.ctl[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
+    s << "    c = uni(c);\n  } else if (aload32(&A.ctl[3]) == 0u) {\n    return;\n  }\n";
+    s << "  u32 walk = 1;\n  bool prev_changed = false;\n";
     s << "  for (;;) {\n";
-    s << "    if (!fix && c >= A.n_chunks) return;\n";
+    s << "    if (!fix) {\n      if (c >= A.n_chunks) return;\n";
+    if (has_state) {
+        // find the next chunk to recompute
+        s << "    } else {\n      u32 nxt = 0xffffffffu;\n";
+        s << "      while (walk < A.n_chunks) {\n";
+        s << "        if (prev_changed) {\n";
+        s << "          const u32 pi = A.chunk_inst ? A.chunk_inst[walk] : 0u;\n";
+        s << "          bool need = false;\n";
+        s << "          if (walk != A.inst[pi].chunk_begin) {\n";
+        s << "            for (u32 i = lane; i < " << n_state << "u; i += 64u)\n";
+        s << "              if (aload64(&A.st_u[(u64)walk * " << n_state << "ull + i]) != aload64(&A.st_t[(u64)(walk - 1) * "
+          << n_state << "ull + i])) need = true;\n";
+        s << "            need = ballot(need) != 0ull;\n";
+        s << "          }\n";
+        s << "          if (need) { nxt = walk; break; }\n";
+        s << "          prev_changed = false;\n          ++walk;\n          continue;\n";
+        s << "        }\n";
+        s << "        const u32 x = walk + lane;\n";
+        s << "        const u64 hits = ballot(x < A.n_chunks && aload32(&A.mismatch[x]) != 0u);\n";
+        s << "        if (hits) { nxt = walk + (u32)__builtin_ctzll(hits); break; }\n";
+        s << "        walk += 64u;\n";
+        s << "      }\n";
+        s << "      if (nxt == 0xffffffffu) return;\n";
+        s << "      c = nxt;\n";
+    }
+    s << "    }\n";
     s << "    const u32 ii = A.chunk_inst ? A.chunk_inst[c] : 0u;\n";
     s << "    const InstDesc D = A.inst[ii];\n";
     s << "    const u32 cc = c - D.chunk_begin;\n";
@@ -282,7 +319,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
             s << "      }\n";
         }
         s << "      changed = ballot(changed) != 0ull;\n";
-        s << "      fence();\n";
+        s << "      drain();\n";
         s << "      if (lane == 0) astore32(&A.st_flag[c], tag);\n";
         s << "    }\n";
         // main mode: verify against the predecessor's published end state
@@ -290,7 +327,6 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
         s << "      bool bad = false;\n";
         s << "      if (cc > 0) {\n";
         s << "        if (lane == 0) { while (aload32(&A.st_flag[c - 1]) != tag) __builtin_amdgcn_s_sleep(1); }\n";
-        s << "        fence();\n";
         s << "        for (u32 i = lane; i < " << n_state << "u; i += 64u)\n";
         s << "          if (aload64(&A.st_u[(u64)c * " << n_state << "ull + i]) != aload64(&A.st_t[(u64)(c - 1) * "
           << n_state << "ull + i])) bad = true;\n";
@@ -303,49 +339,10 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
 
     // next job
     s << "    if (!fix) {\n";
-    s << "      fence();\n";
-    s << "      u32 d = 0;\n      if (lane == 0) d = __hip_atomic_fetch_add(&A.ctl[1], 1u, __ATOMIC_RELAXED, AGENT);\n";
-    s << "      d = uni(d);\n";
-    s << "      if (d != A.n_chunks - 1u) {\n";
-    s << "        if (lane == 0) c = __hip_atomic_fetch_add(&A.ctl[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
-    s << "        c = uni(c);\n        continue;\n      }\n";
-    s << "      fence();\n";
-    if (has_state) {
-        s << "      if (aload32(&A.ctl[3]) == 0u) break;\n";
-        s << "      fix = true;\n      walk = 1u;\n      prev_changed = false;\n";
-    } else {
-        s << "      break;\n";
-    }
-    if (has_state) {
-        s << "    } else {\n";
-        s << "      prev_changed = changed;\n      walk = c + 1u;\n";
-    }
+    s << "      if (lane == 0) c = __hip_atomic_fetch_add(&A.ctl[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
+    s << "      c = uni(c);\n";
+    if (has_state) s << "    } else {\n      prev_changed = changed;\n      walk = c + 1u;\n";
     s << "    }\n";
-    if (has_state) {
-        // find the next chunk to recompute, in order
-        s << "    {\n      u32 nxt = 0xffffffffu;\n";
-        s << "      while (walk < A.n_chunks) {\n";
-        s << "        if (prev_changed) {\n";
-        s << "          const u32 pi = A.chunk_inst ? A.chunk_inst[walk] : 0u;\n";
-        s << "          bool need = false;\n";
-        s << "          if (walk != A.inst[pi].chunk_begin) {\n";
-        s << "            for (u32 i = lane; i < " << n_state << "u; i += 64u)\n";
-        s << "              if (aload64(&A.st_u[(u64)walk * " << n_state << "ull + i]) != aload64(&A.st_t[(u64)(walk - 1) * "
-          << n_state << "ull + i])) need = true;\n";
-        s << "            need = ballot(need) != 0ull;\n";
-        s << "          }\n";
-        s << "          if (need) { nxt = walk; break; }\n";
-        s << "          prev_changed = false;\n          ++walk;\n          continue;\n";
-        s << "        }\n";
-        s << "        const u32 x = walk + lane;\n";
-        s << "        const u64 hits = ballot(x < A.n_chunks && aload32(&A.mismatch[x]) != 0u);\n";
-        s << "        if (hits) { nxt = walk + (u32)__builtin_ctzll(hits); break; }\n";
-        s << "        walk += 64u;\n";
-        s << "      }\n";
-        s << "      if (nxt == 0xffffffffu) break;\n";
-        s << "      c = nxt;\n";
-        s << "    }\n";
-    }
     s << "  }\n";  // job loop
     s << "}\n";
     // The control block and the popcount accumulators are reset by the plan's

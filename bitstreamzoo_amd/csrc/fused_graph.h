@@ -16,10 +16,11 @@
 //     chunk other than the first starts `warmup_rows` rows early with a zero
 //     state (speculation), records the state it reaches at its first row (U)
 //     and publishes the state at its end (T).  Chunk c was exact iff
-//     U[c] == T[c-1]; each chunk checks that against its predecessor, and the
-//     last wave of the launch recomputes, in order, every chunk whose check
-//     failed from its predecessor's true end state.  The result is exact for
-//     every input; speculation only decides how much work is redone;
+//     U[c] == T[c-1]; each chunk checks that against its predecessor, and a
+//     second launch of the same kernel (one wave, mode 1) recomputes, in
+//     order, every chunk whose check failed from its predecessor's true end
+//     state.  The result is exact for every input; speculation only decides
+//     how much work is redone;
 //   * an advance of a value computed by an earlier kernel (any k) and `>>` are
 //     direct loads at the shifted word offset, needing no state;
 //   * popcounts accumulate per lane (v_bcnt), are reduced per wave and added
@@ -114,6 +115,7 @@ struct KArgs {
     unsigned* ctl;                    // ticket, done, epoch, mismatches, recomputed
     uint32_t n_chunks;
     uint32_t n_inst;
+    uint32_t mode;                    // 0 main launch, 1 in-order fix-up (one wave)
 };
 
 }  // namespace bsz

@@ -192,7 +192,9 @@ __global__ __launch_bounds__(256) void k_add(uint64_t* __restrict__ dst, const u
 
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
+        // Every block took its ticket before it counts itself done, so the
+        // last one can reset the counters; statuses are device-scope atomics
+        // and need no cache writeback.
         if (atomicAdd(&ctl[1], 1u) == n_tiles - 1) {
             __hip_atomic_store(&ctl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&ctl[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
