@@ -28,7 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from bitstreamzoo_amd import api, workloads  # noqa: E402
+from bitstreamzoo_amd import api, tuning, workloads  # noqa: E402
 from bitstreamzoo_amd.bsdata import BsCase  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
@@ -43,71 +43,27 @@ def words_popcount(w: np.ndarray) -> int:
     return int(np.unpackbits(w.view(np.uint8)).sum())
 
 
-def shape_case(w: workloads.Workload) -> api.Case:
-    c0 = w.cases[0]
-    small = BsCase(bitlength=64, params=dict(c0.params),
-                   inputs={k: np.zeros(1, np.uint64) for k in c0.inputs},
-                   input_arrays={k: {i: np.zeros(1, np.uint64) for i in a} for k, a in c0.input_arrays.items()})
-    return api.Case.from_bscase(small)
-
-
-class Bound:
-    """A plan with every instance's inputs uploaded and bound."""
-
-    def __init__(self, ctx, w, cases, options=None):
-        self.ctx = ctx
-        self.plan = api.Plan(ctx, api.Program(w.source), shape_case(w), options or {})
-        self.plan.set_instances([c.bitlength for c in cases])
-        self.bufs = []
-        for i, c in enumerate(cases):
-            for n, words in c.inputs.items():
-                d = ctx.upload(words, c.bitlength)
-                self.bufs.append(d)
-                self.plan.bind(i, n, d)
-        self.plan.prepare()
-
-    def free(self):
-        self.plan.close()
-        for d in self.bufs:
-            d.free()
-
-
-def time_graph(ctx, graph, steps, warmup, reps_per_launch=1):
-    graph.launch(warmup)
-    ctx.sync()
-    ctx.timer_start()
-    graph.launch(steps)
-    ms = ctx.timer_stop()
-    return ms / (steps * reps_per_launch)
+def time_plans(ctx, plans, steps, warmup, executor, launches_per_run=1):
+    return tuning.time_runs(ctx, plans, steps, warmup, executor, MAX_GRAPH_LAUNCHES, launches_per_run)
 
 
 MAX_GRAPH_LAUNCHES = 20000
 
 
-def time_plans(ctx, plans, steps, warmup, executor, launches_per_run=1):
-    """Time `steps` runs of every plan.  Runs are captured into one hipGraph
-    and replayed, unless a run enqueues more than MAX_GRAPH_LAUNCHES kernels
-    (the per-op executor over a large batch), which is timed eagerly.
-    Returns ms per plan run."""
-    for p in plans:
-        p.set_executor(executor)
-        p.prepare()
-        p.run()
-    ctx.sync()
-    if launches_per_run * len(plans) > MAX_GRAPH_LAUNCHES:
-        for _ in range(warmup):
-            for p in plans:
-                p.run()
-        ctx.sync()
-        ctx.timer_start()
-        for _ in range(steps):
-            for p in plans:
-                p.run()
-        return ctx.timer_stop() / (steps * len(plans))
-    g = ctx.capture(lambda: [p.run() for p in plans])
-    ms = time_graph(ctx, g, steps, warmup, reps_per_launch=len(plans))
-    del g
-    return ms
+def tuned_options(ctx, args, name, program, residents, kind, int_names, out_names, default):
+    """The autotuner's pick over `kind`'s candidate grid (or `default` with
+    --no-tune), and the tuning record."""
+    if args.schedule:
+        return json.loads(args.schedule), None
+    if args.no_tune:
+        return default, None
+    log(f"{name}: autotuning ({kind} grid)")
+    space = tuning.default_space(kind)
+    tr = tuning.autotune(ctx, program, residents, space, int_names, out_names, steps=max(3, args.steps // 2),
+                         warmup=1, log=log)
+    best = tr.best
+    log(f"{name}: best {best.options} {best.ms:.4f} ms ({tr.seconds:.1f} s, {len(tr.trials)} candidates)")
+    return best.options, tr.as_json()
 
 
 # ── full-size host checks (independent of the executors) ──
@@ -146,36 +102,59 @@ def shift_and_expected_popcount(case: BsCase, pattern: str) -> int:
 
 # ── workloads ──
 
+class Bound:
+    """One plan per resident input set, built with `options`."""
+
+    def __init__(self, ctx, program, residents, options):
+        self.plans = []
+        for r in residents:
+            p = api.Plan(ctx, program, tuning.shape_case(r.cases), options)
+            r.bind(p)
+            self.plans.append(p)
+
+    def free(self):
+        for p in self.plans:
+            p.close()
+
+
 def bench_adder(ctx, args, perop=True):
     bl = args.bitlength or (1 << 28)
     log(f"adder32: generating 65 x {bl} bits")
     w = workloads.adder32(bl, seed=2)
     case = w.cases[0]
-    B = Bound(ctx, w, [case])
-    B.plan.run()
+    prog = api.Program(w.source)
+    res = tuning.Resident(ctx, [case])
+    outs = [f"s{i}" for i in range(32)] + ["cout"]
+    opts, tune = tuned_options(ctx, args, "adder32", prog, [res], "wide", ["n_" + o for o in outs], outs,
+                               {"rows_per_step": 1, "occupancy": 2})
+    B = Bound(ctx, prog, [res], opts)
+    plan = B.plans[0]
+    plan.run()
     ctx.sync()
     log("adder32: checking all 33 outputs and popcounts against the host computation")
     exp = adder_expected(case, 32)
     nw = len(case.inputs["a0"])
     ok = True
     for name, e in exp.items():
-        got = B.plan.download_output(0, name, nw)
+        got = plan.download_output(0, name, nw)
         if not np.array_equal(got, e):
             ok = False
             log(f"MISMATCH in output {name}")
-        if B.plan.read_int(0, "n_" + name) != words_popcount(e):
+        if plan.read_int(0, "n_" + name) != words_popcount(e):
             ok = False
             log(f"MISMATCH in popcount n_{name}")
     if not ok:
         raise SystemExit("adder32 outputs differ from the host computation")
-    ms = time_plans(ctx, [B.plan], args.steps, args.warmup, api.FUSED)
+    ms = time_plans(ctx, [plan], args.steps, args.warmup, api.FUSED)
     log(f"adder32: fused {ms:.4f} ms/step")
-    res = {"ms_per_step": ms, "bit_ops": w.bit_ops, "alg_bytes": w.alg_bytes, "plan": B.plan.info()}
+    out = {"ms_per_step": ms, "bit_ops": w.bit_ops, "alg_bytes": w.alg_bytes, "plan": plan.info(),
+           "options": opts, "tune": tune}
     if perop:
-        res["perop_ms_per_step"] = time_plans(ctx, [B.plan], max(2, args.steps // 4), 1, api.PEROP)
-        log(f"adder32: per-op {res['perop_ms_per_step']:.4f} ms/step")
+        out["perop_ms_per_step"] = time_plans(ctx, [plan], max(2, args.steps // 4), 1, api.PEROP)
+        log(f"adder32: per-op {out['perop_ms_per_step']:.4f} ms/step")
     B.free()
-    return w, res
+    res.free()
+    return w, out
 
 
 def bench_matchstar(ctx, args):
@@ -183,50 +162,61 @@ def bench_matchstar(ctx, args):
     rot = 12  # distinct input sets, so the 256 MB last-level cache cannot serve them
     log(f"matchstar: generating {rot} x 2 x {bl} bits")
     ws = [workloads.matchstar(bl, seed=100 + r) for r in range(rot)]
-    bound = [Bound(ctx, w, w.cases) for w in ws]
-    for B, w in zip(bound, ws):
-        B.plan.run()
+    prog = api.Program(ws[0].source)
+    residents = [tuning.Resident(ctx, w.cases) for w in ws]
+    opts, tune = tuned_options(ctx, args, "matchstar", prog, residents, "stateful", ["n"], ["result"],
+                               {"chunk_rows": 16, "rows_per_step": 8})
+    B = Bound(ctx, prog, residents, opts)
+    for p in B.plans:
+        p.run()
     ctx.sync()
-    for B, w in zip(bound, ws):
+    for p, w in zip(B.plans, ws):
         e = matchstar_expected(w.cases[0])
-        got = B.plan.download_output(0, "result", len(e))
-        if not np.array_equal(got, e) or B.plan.read_int(0, "n") != words_popcount(e):
+        got = p.download_output(0, "result", len(e))
+        if not np.array_equal(got, e) or p.read_int(0, "n") != words_popcount(e):
             raise SystemExit("matchstar output differs from the host computation")
-    ms = time_plans(ctx, [b.plan for b in bound], args.steps, args.warmup, api.FUSED)
+    ms = time_plans(ctx, B.plans, args.steps, args.warmup, api.FUSED)
     log(f"matchstar: fused {ms:.4f} ms/step")
-    perop = time_plans(ctx, [b.plan for b in bound], max(2, args.steps // 4), 1, api.PEROP)
+    perop = time_plans(ctx, B.plans, max(2, args.steps // 4), 1, api.PEROP)
     log(f"matchstar: per-op {perop:.4f} ms/step")
-    info = bound[0].plan.info()
-    for B in bound:
-        B.free()
+    info = B.plans[0].info()
+    B.free()
+    for r in residents:
+        r.free()
     return ws[0], {"ms_per_step": ms, "perop_ms_per_step": perop, "bit_ops": ws[0].bit_ops,
-                   "alg_bytes": ws[0].alg_bytes, "plan": info, "input_sets": rot}
+                   "alg_bytes": ws[0].alg_bytes, "plan": info, "input_sets": rot, "options": opts, "tune": tune}
 
 
 def bench_shift_and(ctx, args):
     n = 2048
     log(f"shift_and: generating {n} sequences, lengths log-uniform in [2^12, 2^22]")
     w = workloads.shift_and_batch(n_seqs=n, seed=3)
-    B = Bound(ctx, w, w.cases, {"chunk_rows": 16, "rows_per_step": 4})
-    log(f"shift_and: plan bound over {n} instances ({B.plan.info()['n_chunks']} chunks); running")
-    B.plan.run()
+    prog = api.Program(w.source)
+    res = tuning.Resident(ctx, w.cases)
+    opts, tune = tuned_options(ctx, args, "shift_and", prog, [res], "small", ["n"], ["match"],
+                               {"chunk_rows": 16, "rows_per_step": 4})
+    B = Bound(ctx, prog, [res], opts)
+    plan = B.plans[0]
+    log(f"shift_and: plan bound over {n} instances ({plan.info()['n_chunks']} chunks); running")
+    plan.run()
     ctx.sync()
     pattern = w.source.split("the 32-mer ")[1].split(":")[0]
     rng = np.random.Generator(np.random.PCG64(7))
     check_idx = sorted(set(rng.integers(0, n, size=16).tolist()) | {0, n - 1})
     for i in check_idx:
-        if B.plan.read_int(i, "n") != shift_and_expected_popcount(w.cases[i], pattern):
+        if plan.read_int(i, "n") != shift_and_expected_popcount(w.cases[i], pattern):
             raise SystemExit(f"shift_and popcount of sequence {i} differs from the host computation")
     log("shift_and: outputs checked; timing the fused plan")
-    ms = time_plans(ctx, [B.plan], args.steps, args.warmup, api.FUSED)
+    ms = time_plans(ctx, [plan], args.steps, args.warmup, api.FUSED)
     log(f"shift_and: fused {ms:.4f} ms/step; timing the per-op executor (eager launches)")
-    ops_per_inst = B.plan.info()["graph_nodes"]
-    perop = time_plans(ctx, [B.plan], 2, 1, api.PEROP, launches_per_run=n * ops_per_inst)
+    ops_per_inst = plan.info()["graph_nodes"]
+    perop = time_plans(ctx, [plan], 2, 1, api.PEROP, launches_per_run=n * ops_per_inst)
     log(f"shift_and: per-op {perop:.4f} ms/step")
-    info = B.plan.info()
+    info = plan.info()
     B.free()
+    res.free()
     return w, {"ms_per_step": ms, "perop_ms_per_step": perop, "bit_ops": w.bit_ops, "alg_bytes": w.alg_bytes,
-               "plan": info, "instances": n, "checked_instances": len(check_idx)}
+               "plan": info, "instances": n, "checked_instances": len(check_idx), "options": opts, "tune": tune}
 
 
 def copy_bandwidth(ctx) -> float:
@@ -235,7 +225,11 @@ def copy_bandwidth(ctx) -> float:
     src = ctx.alloc(bl)
     dst = ctx.alloc(bl)
     g = ctx.capture(lambda: api.lib().bsz_op_copy(ctx.h, dst.dev, src.dev, bl))
-    ms = time_graph(ctx, g, 10, 2)
+    g.launch(2)
+    ctx.sync()
+    ctx.timer_start()
+    g.launch(10)
+    ms = ctx.timer_stop() / 10
     src.free()
     dst.free()
     return 2 * (bl // 8) / (ms * 1e-3) / 1e9
@@ -271,6 +265,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perop", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner; use fixed default schedules")
+    ap.add_argument("--schedule", default="", help="JSON options for the measured workload's plan (skips tuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the executor runs on a single MI355X and this build has no "
@@ -319,11 +315,13 @@ def main():
             "matchstar": "MatchStar on 2 x 2^26-bit streams + popcount, 12 rotating input sets",
             "shift_and": "batched Shift-And, 2048 sequences, 32-mer, one launch"}[args.workload],
             "executor": "fused", "bitlength": w.cases[0].bitlength if args.workload != "shift_and" else None,
-            "kernels": len(r["plan"]["kernels"]), "chunk_rows": r["plan"]["chunk_rows"]},
+            "kernels": len(r["plan"]["kernels"]), "schedule": r["options"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
                      "measured_copy_GBps": round(copy_gbps, 1)},
     }
+    if r.get("tune"):
+        line["autotune"] = r["tune"]
     if "perop_ms_per_step" in r:
         line["perop"] = {"ms_per_step": round(r["perop_ms_per_step"], 5),
                          "value": round(r["bit_ops"] / (r["perop_ms_per_step"] * 1e-3) / 1e9, 1),
@@ -337,7 +335,8 @@ def main():
         ex[name] = {"ms_per_step": round(m, 5), "value": round(rr["bit_ops"] / (m * 1e-3) / 1e9, 1),
                     "GBps": round(rr["alg_bytes"] / (m * 1e-3) / 1e9, 1),
                     "hbm_frac": round(rr["alg_bytes"] / (m * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-                    "perop_ms_per_step": round(rr["perop_ms_per_step"], 5),
+                    "perop_ms_per_step": round(rr["perop_ms_per_step"], 5), "schedule": rr["options"],
+                    "tuning_s": rr["tune"]["tuning_s"] if rr.get("tune") else None,
                     "speedup_fused_over_perop": round(rr["perop_ms_per_step"] / m, 2)}
     if ex:
         line["other_configs"] = ex

@@ -298,8 +298,11 @@ class Plan:
         check(lib().bsz_plan_output(self.h, inst, name.encode(), idx, C.byref(p)), self.ctx.h)
         return p
 
-    def download_output(self, inst: int, name: str, n_words: int, idx: int = -1) -> np.ndarray:
+    def download_output(self, inst: int, name: str, n_words: int, idx: int = -1, offset: int = 0) -> np.ndarray:
+        """Words [offset, offset + n_words) of an output stream."""
         p = self.output_ptr(inst, name, idx)
+        if offset:
+            p = C.cast(C.c_void_p(C.cast(p, C.c_void_p).value + 8 * offset), _lib.u64p)
         out = np.zeros(n_words, dtype=np.uint64)
         check(lib().bsz_d2h(self.ctx.h, ptr(out), p, n_words), self.ctx.h)
         return out

@@ -41,6 +41,7 @@ FusedOptions FusedOptions::from_env() {
     o.rows_per_step = env_int("BSZ_ROWS_PER_STEP", o.rows_per_step);
     o.max_ops = env_int("BSZ_MAX_OPS", o.max_ops);
     o.max_live = env_int("BSZ_MAX_LIVE", o.max_live);
+    o.occupancy = env_int("BSZ_OCCUPANCY", o.occupancy);
     if (const char* d = std::getenv("BSZ_CACHE_DIR")) o.cache_dir = d;
     o.verbose = env_int("BSZ_VERBOSE", 0) != 0;
     return o;
@@ -59,6 +60,7 @@ FusedOptions FusedOptions::from_json(const std::string& text) {
     get("rows_per_step", o.rows_per_step);
     get("max_ops", o.max_ops);
     get("max_live", o.max_live);
+    get("occupancy", o.occupancy);
     if (const Json* v = j.get("cache_dir")) o.cache_dir = v->s;
     return o;
 }
@@ -68,9 +70,10 @@ PlanOptions to_plan_options(const FusedOptions& f) {
     p.chunk_rows = std::max(1, f.chunk_rows);
     p.warmup_rows = std::max(1, f.warmup_rows);
     p.waves_per_block = std::min(16, std::max(1, f.waves_per_block));
-    p.rows_per_step = std::max(1, std::min(f.rows_per_step, p.chunk_rows));
+    p.rows_per_step = std::max(1, f.rows_per_step);
     p.max_ops = std::max(1, f.max_ops);
     p.max_live = std::max(4, f.max_live);
+    p.occupancy = f.occupancy <= 0 ? 0 : std::min(8, f.occupancy);
     return p;
 }
 
@@ -224,8 +227,7 @@ public:
     // Lays out the device arena for `geoms` (one per instance).
     void configure(const std::vector<Geom>& geoms, hipStream_t s) {
         const PlanSpec& P = *spec;
-        const int U = std::max(1, P.opt.rows_per_step);
-        chunk_rows = (uint64_t)std::max(U, (P.opt.chunk_rows / U) * U);
+        chunk_rows = (uint64_t)std::max(1, P.opt.chunk_rows);
         n_inst = (uint32_t)geoms.size();
         inst_h_.assign(n_inst, InstDesc{});
         uint64_t nc = 0;
@@ -1104,7 +1106,8 @@ std::string FusedPlan::info_json() const {
         const KernelSpec& ks = I.spec->kernels[k];
         o << (k ? "," : "") << "{\"name\":\"" << ks.name << "\",\"nodes\":" << ks.nodes.size()
           << ",\"inputs\":" << ks.inputs.size() << ",\"outputs\":" << ks.outputs.size() << ",\"pops\":" << ks.pops.size()
-          << ",\"state_words\":" << ks.n_state << "}";
+          << ",\"state_words\":" << ks.n_state << ",\"rows_per_step\":" << ks.rows_per_step
+          << ",\"occupancy\":" << ks.occupancy << "}";
     }
     o << "],\"sources\":[";
     for (size_t j = 0; j < I.sources.size(); ++j)
@@ -1119,6 +1122,7 @@ std::string FusedPlan::info_json() const {
       << ",\"chunk_rows\":" << I.spec->opt.chunk_rows << ",\"warmup_rows\":" << I.spec->opt.warmup_rows
       << ",\"waves_per_block\":" << I.spec->opt.waves_per_block << ",\"rows_per_step\":" << I.spec->opt.rows_per_step
       << ",\"max_ops\":" << I.spec->opt.max_ops << ",\"max_live\":" << I.spec->opt.max_live
+      << ",\"occupancy\":" << I.spec->opt.occupancy
       << ",\"n_chunks\":" << (I.rt ? I.rt->n_chunks : 0) << ",\"device_bytes\":" << (I.rt ? I.rt->device_bytes() : 0)
       << "}";
     return o.str();

@@ -12,9 +12,10 @@ struct FusedOptions {
     int chunk_rows = 16;        // 64-word rows per wave-chunk
     int warmup_rows = 1;        // speculative warm-up rows before a chunk
     int waves_per_block = 4;    // independent wavefronts per workgroup
-    int rows_per_step = 4;      // rows whose loads are issued together
+    int rows_per_step = 8;      // upper bound on rows whose loads are issued together
     int max_ops = 4000;         // fusion depth: DAG nodes per kernel
     int max_live = 96;          // live 64-bit values per lane per kernel
+    int occupancy = 0;          // waves per SIMD the register budget allows (512/occupancy VGPRs); 0 = auto
     std::string cache_dir;      // code-object cache ("" = $BSZ_CACHE_DIR or ~/.cache/bsz_hip)
     bool verbose = false;
     static FusedOptions from_env();

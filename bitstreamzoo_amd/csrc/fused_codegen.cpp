@@ -47,6 +47,22 @@ struct KArgs {
   u32 n_chunks; u32 n_inst; u32 mode;
 };
 #define AGENT __HIP_MEMORY_SCOPE_AGENT
+// Binding tables are never written while a kernel runs: reading them through
+// the constant address space lets the compiler use scalar loads, so stream
+// pointers live in SGPRs (or VGPR lanes) instead of 2 VGPRs each.
+typedef __attribute__((address_space(4))) const void* cptr_t;
+__device__ __forceinline__ const u64* tab_in(const u64* const* t, u64 i) {
+  return ((const u64* __attribute__((address_space(4))) const*)(cptr_t)t)[i];
+}
+__device__ __forceinline__ u64* tab_out(u64* const* t, u64 i) {
+  return ((u64* __attribute__((address_space(4))) const*)(cptr_t)t)[i];
+}
+__device__ __forceinline__ InstDesc inst_at(const InstDesc* t, u32 i) {
+  return ((const InstDesc __attribute__((address_space(4)))*)(cptr_t)t)[i];
+}
+__device__ __forceinline__ u32 u32_at(const u32* t, u32 i) {
+  return ((const u32 __attribute__((address_space(4)))*)(cptr_t)t)[i];
+}
 __device__ __forceinline__ u32 lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 __device__ __forceinline__ u64 ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 __device__ __forceinline__ u32 uni(u32 x) { return (u32)__builtin_amdgcn_readfirstlane((int)x); }
@@ -64,8 +80,28 @@ __device__ __forceinline__ u64 prev_word(u64 x, u64 low) {
 __device__ __forceinline__ u64 word_mask(long long w, u64 lw, u64 lastmask) {
   return (w < 0) ? 0ull : ((u64)w + 1 < lw ? ~0ull : ((u64)w + 1 == lw ? lastmask : 0ull));
 }
-__device__ __forceinline__ u64 ld(const u64* p, long long w, u64 n) {
-  return (w >= 0 && (u64)w < n) ? p[w] : 0ull;
+// A wave-uniform row pointer made opaque to the optimizer, so that every row
+// access is a scalar base plus the lane offset (global_load v, v_off, s[base])
+// rather than a per-lane 64-bit pointer kept per stream.
+#define GAS __attribute__((address_space(1)))
+__device__ __forceinline__ const GAS u64* urow(const u64* p) {
+  const u64 v = (u64)p;
+  return (const GAS u64*)(((u64)uni((u32)(v >> 32)) << 32) | uni((u32)v));
+}
+__device__ __forceinline__ GAS u64* urow_w(u64* p) {
+  const u64 v = (u64)p;
+  return (GAS u64*)(((u64)uni((u32)(v >> 32)) << 32) | uni((u32)v));
+}
+// rowbase[lane] as scalar base + 32-bit byte offset (the saddr form)
+__device__ __forceinline__ u64 lane_ld(const GAS u64* rowbase, u32 off8) {
+  return *(const GAS u64*)((const GAS char*)rowbase + off8);
+}
+__device__ __forceinline__ void lane_st(GAS u64* rowbase, u32 off8, u64 v) {
+  *(GAS u64*)((GAS char*)rowbase + off8) = v;
+}
+// word `w` of a stream of n words (0 outside), read at row base + per-lane offset
+__device__ __forceinline__ u64 ldr(const GAS u64* rowbase, long long off, long long w, u64 n) {
+  return (w >= 0 && (u64)w < n) ? rowbase[off] : 0ull;
 }
 __device__ __forceinline__ u64 fsl(u64 cur, u64 lower, u32 b) { return b ? ((cur << b) | (lower >> (64 - b))) : cur; }
 __device__ __forceinline__ u64 fsr(u64 cur, u64 upper, u32 b) { return b ? ((cur >> b) | (upper << (64 - b))) : cur; }
@@ -117,9 +153,6 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
     }
     ks.n_state = n_state;
     const bool has_state = n_state > 0;
-    const int U = std::max(1, opt.rows_per_step);
-    const int CR = std::max(U, (opt.chunk_rows / U) * U);
-    const int WR = has_state ? std::max(1, opt.warmup_rows) : 0;
 
     // which graph nodes need a plain value of an input (loaded per row)
     std::set<int32_t> plain_inputs;
@@ -142,11 +175,58 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
     for (int32_t n : ks.outputs)
         if (!in_kernel.count(n)) note_operand(n);
 
+    // Rows whose loads are issued together: as many as the option allows while
+    // the prefetched words stay within ~96 VGPRs (2 per 64-bit word), so a
+    // kernel with many inputs does not spill.
+    // Words at a signed offset from the row's own word that external shifts
+    // read: one load per distinct (input, offset), shared by every shift.
+    std::set<std::pair<int32_t, long long>> off_words;
+    for (int32_t n : ext_shifts) {
+        const GNode& x = g.nodes[n];
+        const GNode& src = g.nodes[x.a];
+        if (src.op == NOp::Zero || src.op == NOp::Ones) continue;
+        const long long q = (long long)(x.k / 64);
+        const bool part = (x.k % 64) != 0;
+        if (x.op == NOp::Shl) {
+            if (q == 0) plain_inputs.insert(x.a); else off_words.insert({x.a, -q});
+            if (part) off_words.insert({x.a, -q - 1});
+        } else {
+            if (q == 0) plain_inputs.insert(x.a); else off_words.insert({x.a, q});
+            if (part) off_words.insert({x.a, q + 1});
+        }
+    }
+    // The chunk length is the plan's, shared by all its kernels; U divides it.
+    const int loads_per_row = (int)(plain_inputs.size() + off_words.size());
+    const int CR = std::max(1, opt.chunk_rows);
+    // With an explicit occupancy the prefetch may fill that register budget.
+    const int NPOP_ = (int)ks.pops.size();
+    const int live_ = std::min(opt.max_live, (int)ks.nodes.size());
+    const int load_budget =
+        opt.occupancy > 0 ? std::max(0, 512 / opt.occupancy - 24 - NPOP_ - live_) : 96;
+    const int u_cap = std::max(1, std::min(opt.rows_per_step, load_budget / std::max(1, 2 * loads_per_row)));
+    int U = 1;
+    for (int d = 1; d <= std::min(u_cap, CR); ++d)
+        if (CR % d == 0) U = d;
+    const int WR = has_state ? std::max(1, opt.warmup_rows) : 0;
+    ks.rows_per_step = U;
+    ks.chunk_rows = CR;
+
     std::ostringstream s;
     const int NIN = (int)ks.inputs.size(), NOUT = (int)ks.outputs.size(), NPOP = (int)ks.pops.size();
-    s << "extern \"C\" __global__ __launch_bounds__(" << 64 * opt.waves_per_block << ") void " << ks.name
-      << "(KArgs A) {\n";
+    // Register budget: at least `occupancy` waves per SIMD (512/occupancy VGPRs),
+    // which also bounds how many row loads the scheduler hoists.
+    // estimated VGPRs: prefetched words, popcount accumulators, live values
+    const int est_vgprs = 2 * loads_per_row * U + NPOP + 2 * std::min(opt.max_live, (int)ks.nodes.size()) / 2 + 24;
+    int occ = opt.occupancy;
+    if (occ <= 0) {
+        occ = 4;
+        while (occ > 1 && est_vgprs > 512 / occ) --occ;
+    }
+    ks.occupancy = occ;
+    s << "extern \"C\" __global__ __launch_bounds__(" << 64 * opt.waves_per_block
+      << ") __attribute__((amdgpu_waves_per_eu(" << occ << "))) void " << ks.name << "(KArgs A) {\n";
     s << "  const u32 lane = lane_id();\n";
+    s << "  const u32 off8 = lane * 8u;\n";
     s << "  const u32 tag = aload32(&A.ctl[2]) + 1u;\n";
     // mode 0: the main launch, a queue of chunks; mode 1: one wave that
     // recomputes, in order, the chunks whose speculative start state was wrong
@@ -162,9 +242,9 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
         s << "    } else {\n      u32 nxt = 0xffffffffu;\n";
         s << "      while (walk < A.n_chunks) {\n";
         s << "        if (prev_changed) {\n";
-        s << "          const u32 pi = A.chunk_inst ? A.chunk_inst[walk] : 0u;\n";
+        s << "          const u32 pi = A.chunk_inst ? u32_at(A.chunk_inst, walk) : 0u;\n";
         s << "          bool need = false;\n";
-        s << "          if (walk != A.inst[pi].chunk_begin) {\n";
+        s << "          if (walk != inst_at(A.inst, pi).chunk_begin) {\n";
         s << "            for (u32 i = lane; i < " << n_state << "u; i += 64u)\n";
         s << "              if (aload64(&A.st_u[(u64)walk * " << n_state << "ull + i]) != aload64(&A.st_t[(u64)(walk - 1) * "
           << n_state << "ull + i])) need = true;\n";
@@ -182,12 +262,12 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
         s << "      c = nxt;\n";
     }
     s << "    }\n";
-    s << "    const u32 ii = A.chunk_inst ? A.chunk_inst[c] : 0u;\n";
-    s << "    const InstDesc D = A.inst[ii];\n";
+    s << "    const u32 ii = A.chunk_inst ? u32_at(A.chunk_inst, c) : 0u;\n";
+    s << "    const InstDesc D = inst_at(A.inst, ii);\n";
     s << "    const u32 cc = c - D.chunk_begin;\n";
     s << "    const u64 NW = D.rows * 64ull;\n";
-    for (int j = 0; j < NIN; ++j) s << "    const u64* __restrict__ in" << j << " = A.in[(u64)ii * " << NIN << "u + " << j << "u];\n";
-    for (int j = 0; j < NOUT; ++j) s << "    u64* __restrict__ out" << j << " = A.out[(u64)ii * " << NOUT << "u + " << j << "u];\n";
+    for (int j = 0; j < NIN; ++j) s << "    const u64* __restrict__ in" << j << " = tab_in(A.in, (u64)ii * " << NIN << "u + " << j << "u);\n";
+    for (int j = 0; j < NOUT; ++j) s << "    u64* __restrict__ out" << j << " = tab_out(A.out, (u64)ii * " << NOUT << "u + " << j << "u);\n";
     s << "    const u64 r_begin = (u64)cc * " << CR << "ull;\n";
     s << "    const u64 r_end = (r_begin + " << CR << "ull < D.rows) ? r_begin + " << CR << "ull : D.rows;\n";
     if (has_state) {
@@ -202,24 +282,42 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
     s << "    for (u64 r = r_begin - warm; r < r_end; r += " << U << "ull) {\n";
     // load phase
     for (int u = 0; u < U; ++u) {
-        s << "      const long long w" << u << " = (long long)((r + " << u << "ull) * 64ull + lane);\n";
-        s << "      const bool ok" << u << " = r + " << u << "ull < r_end;\n";
-        for (int32_t a : plain_inputs)
-            s << "      const u64 x" << a << "_" << u << " = ok" << u << " ? in" << input_index.at(a) << "[w" << u << "] : 0ull;\n";
+        // The row index goes through readfirstlane so that addresses are a
+        // scalar row base plus the lane offset (no per-lane pointer per stream).
+        s << "      const u64 ru" << u << " = (u64)uni((u32)(r + " << u << "ull));\n";
+        s << "      const long long w" << u << " = (long long)(ru" << u << " * 64ull + lane);\n";
+        if (U > 1) s << "      const bool ok" << u << " = r + " << u << "ull < r_end;\n";
+        else s << "      const bool ok" << u << " = true;\n";
+        // row base (wave-uniform, scalar) + lane: loads take the SGPR-base +
+        // VGPR-offset form, so no per-lane pointer is kept per stream
+        for (int32_t a : plain_inputs) {
+            const std::string rb = "urow(in" + std::to_string(input_index.at(a)) + " + ru" + std::to_string(u) + " * 64ull)";
+            if (U > 1)
+                s << "      const u64 x" << a << "_" << u << " = ok" << u << " ? lane_ld(" << rb << ", off8) : 0ull;\n";
+            else
+                s << "      const u64 x" << a << "_" << u << " = lane_ld(" << rb << ", off8);\n";
+        }
+        for (auto& ow : off_words) {
+            const std::string d = ow.second < 0 ? " - " + std::to_string(-ow.second) + "ll" : " + " + std::to_string(ow.second) + "ll";
+            s << "      const u64 z" << ow.first << "_" << (ow.second < 0 ? "m" : "p") << std::llabs(ow.second) << "_" << u
+              << " = ok" << u << " ? ldr(urow(in" << input_index.at(ow.first) << " + ru" << u << " * 64ull), (long long)lane"
+              << d << ", w" << u << d << ", NW) : 0ull;\n";
+        }
         for (int32_t n : ext_shifts) {
             const GNode& x = g.nodes[n];
-            const uint64_t q = x.k / 64;
+            const long long q = (long long)(x.k / 64);
             const unsigned b = (unsigned)(x.k % 64);
             const GNode& src = g.nodes[x.a];
-            auto load = [&](const std::string& off) -> std::string {
+            auto word = [&](long long d) -> std::string {
                 if (src.op == NOp::Zero) return "0ull";
-                if (src.op == NOp::Ones) return "word_mask(w" + std::to_string(u) + off + ", D.lw, D.lastmask)";
-                return "ld(in" + std::to_string(input_index.at(x.a)) + ", w" + std::to_string(u) + off + ", NW)";
+                std::string wd = "w" + std::to_string(u) + (d < 0 ? " - " + std::to_string(-d) + "ll" : " + " + std::to_string(d) + "ll");
+                if (src.op == NOp::Ones) return "word_mask(" + wd + ", D.lw, D.lastmask)";
+                if (d == 0) return "x" + std::to_string(x.a) + "_" + std::to_string(u);
+                return "z" + std::to_string(x.a) + "_" + (d < 0 ? "m" : "p") + std::to_string(std::llabs(d)) + "_" + std::to_string(u);
             };
-            std::string qs = std::to_string(q), q1 = std::to_string(q + 1);
             std::string e = x.op == NOp::Shl
-                ? "fsl(" + load(" - " + qs + "ll") + ", " + load(" - " + q1 + "ll") + ", " + std::to_string(b) + "u)"
-                : "fsr(" + load(" + " + qs + "ll") + ", " + load(" + " + q1 + "ll") + ", " + std::to_string(b) + "u)";
+                ? "fsl(" + word(-q) + ", " + (b ? word(-q - 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)"
+                : "fsr(" + word(q) + ", " + (b ? word(q + 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)";
             s << "      const u64 y" << n << "_" << u << " = ok" << u << " ? " << e << " : 0ull;\n";
         }
     }
@@ -240,6 +338,24 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
             if (x.op == NOp::Zero) return "0ull";
             if (x.op == NOp::Ones) return "m";
             return "x" + std::to_string(a) + "_" + std::to_string(u);
+        };
+        // outputs and popcounts of each node are emitted right after it, so
+        // its register is free again at once
+        std::map<int32_t, std::vector<int>> out_of, pop_of;
+        for (int j = 0; j < NOUT; ++j) out_of[ks.outputs[j]].push_back(j);
+        for (int p = 0; p < NPOP; ++p) pop_of[g.pops[ks.pops[p]]].push_back(p);
+        const bool guard = has_state;
+        auto sink = [&](int32_t n) {
+            auto oi = out_of.find(n);
+            auto pi = pop_of.find(n);
+            if (oi == out_of.end() && pi == pop_of.end()) return;
+            if (guard) s << "        if (r + " << u << "ull >= r_begin) {\n";
+            if (oi != out_of.end())
+                for (int j : oi->second)
+                    s << "          lane_st(urow_w(out" << j << " + ru" << u << " * 64ull), off8, " << ref(n) << " & m);\n";
+            if (pi != pop_of.end())
+                for (int p : pi->second) s << "          pc" << p << " += (u32)__builtin_popcountll(" << ref(n) << " & m);\n";
+            if (guard) s << "        }\n";
         };
         for (int32_t n : ks.nodes) {
             const GNode& x = g.nodes[n];
@@ -279,14 +395,16 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
                 }
                 case NOp::Input: break;
             }
+            sink(n);
         }
-        const bool guard = has_state;
-        if (guard) s << "        if (r + " << u << "ull >= r_begin) {\n";
-        for (int j = 0; j < NOUT; ++j)
-            s << "          out" << j << "[w" << u << "] = " << ref(ks.outputs[j]) << " & m;\n";
-        for (int p = 0; p < NPOP; ++p)
-            s << "          pc" << p << " += (u32)__builtin_popcountll(" << ref(g.pops[ks.pops[p]]) << " & m);\n";
-        if (guard) s << "        }\n";
+        // outputs / popcounts of values this kernel does not compute (inputs, constants)
+        {
+            std::set<int32_t> done(ks.nodes.begin(), ks.nodes.end());
+            std::set<int32_t> rest;
+            for (auto& kv : out_of) if (!done.count(kv.first)) rest.insert(kv.first);
+            for (auto& kv : pop_of) if (!done.count(kv.first)) rest.insert(kv.first);
+            for (int32_t n : rest) sink(n);
+        }
         s << "      }\n";
     }
     s << "    }\n";  // row loop
@@ -367,7 +485,7 @@ std::string SegGraph::key() const {
 std::string PlanOptions::key() const {
     std::ostringstream s;
     s << chunk_rows << "," << warmup_rows << "," << waves_per_block << "," << rows_per_step << "," << max_ops << ","
-      << max_live;
+      << max_live << "," << occupancy;
     return s.str();
 }
 

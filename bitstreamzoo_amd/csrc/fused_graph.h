@@ -54,9 +54,10 @@ struct PlanOptions {
     int chunk_rows = 16;
     int warmup_rows = 1;
     int waves_per_block = 4;
-    int rows_per_step = 4;           // rows whose loads are issued together
+    int rows_per_step = 8;           // upper bound on rows whose loads are issued together
     int max_ops = 4000;
     int max_live = 96;
+    int occupancy = 0;               // waves per SIMD the register budget allows; 0 = from an estimate
     std::string key() const;
 };
 
@@ -75,6 +76,9 @@ struct KernelSpec {
     std::vector<int32_t> out_slot;   // per output: -1 segment output, else intermediate slot
     std::vector<int32_t> pops;       // index into SegGraph::pops accumulated here
     int n_state = 0;                 // 64-bit state words
+    int rows_per_step = 1;           // as generated (capped by register budget)
+    int chunk_rows = 16;             // as generated (a multiple of rows_per_step)
+    int occupancy = 4;               // waves per SIMD the register budget was set for
     std::string source;              // generated HIP
 };
 

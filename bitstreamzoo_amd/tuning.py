@@ -1,0 +1,194 @@
+"""Measured autotuner for fused plans.
+
+A schedule is a set of FusedOptions keys (chunk_rows, rows_per_step,
+waves_per_block, occupancy, warmup_rows, max_ops, max_live): the rows a wave
+walks per chunk, the rows whose loads are issued together (words per lane in
+flight), the workgroup width, the register budget and the fusion depth.  The
+tuner builds one plan per candidate over the same resident inputs, checks
+that every candidate produces the same popcounts and the same output words as
+the first one, and times each with hipEvents over replays of a captured
+hipGraph.  Schedules never change output bits; the check guards that.
+
+The reference has no schedule search (one kernel per op, fixed 256-thread
+blocks: kernels.cuh:7-36, cuda_backend.cuh:53-84).
+"""
+
+from __future__ import annotations
+
+import itertools
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import api
+from .bsdata import BsCase
+
+
+def shape_case(cases: list[BsCase]) -> api.Case:
+    """A 64-bit case with the params and input names of `cases[0]`: enough to
+    record a straight-line program."""
+    c0 = cases[0]
+    small = BsCase(bitlength=64, params=dict(c0.params),
+                   inputs={k: np.zeros(1, np.uint64) for k in c0.inputs},
+                   input_arrays={k: {i: np.zeros(1, np.uint64) for i in a} for k, a in c0.input_arrays.items()})
+    return api.Case.from_bscase(small)
+
+
+class Resident:
+    """Every case's inputs uploaded once to device streams."""
+
+    def __init__(self, ctx: api.Context, cases: list[BsCase]):
+        self.ctx, self.cases = ctx, cases
+        self.bufs: dict[tuple[int, str, int], api.DevStream] = {}
+        for i, c in enumerate(cases):
+            for n, w in c.inputs.items():
+                self.bufs[(i, n, -1)] = ctx.upload(w, c.bitlength)
+            for n, arr in c.input_arrays.items():
+                for j, w in arr.items():
+                    self.bufs[(i, n, int(j))] = ctx.upload(w, c.bitlength)
+
+    def bind(self, plan: api.Plan):
+        plan.set_instances([c.bitlength for c in self.cases])
+        for (i, n, j), d in self.bufs.items():
+            plan.bind(i, n, d, j)
+        plan.prepare()
+
+    def free(self):
+        for d in self.bufs.values():
+            d.free()
+        self.bufs.clear()
+
+
+def time_runs(ctx: api.Context, plans: list[api.Plan], steps: int, warmup: int,
+              executor: int = api.FUSED, max_graph_launches: int = 20000,
+              launches_per_run: int = 1) -> float:
+    """ms per plan run: `steps` runs of every plan, captured into one hipGraph
+    and replayed (or enqueued eagerly when a run holds more than
+    `max_graph_launches` kernels), timed with hipEvents on the context stream."""
+    for p in plans:
+        p.set_executor(executor)
+        p.prepare()
+        p.run()
+    ctx.sync()
+    if launches_per_run * len(plans) > max_graph_launches:
+        for _ in range(warmup):
+            for p in plans:
+                p.run()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(steps):
+            for p in plans:
+                p.run()
+        return ctx.timer_stop() / (steps * len(plans))
+    g = ctx.capture(lambda: [p.run() for p in plans])
+    g.launch(warmup)
+    ctx.sync()
+    ctx.timer_start()
+    g.launch(steps)
+    ms = ctx.timer_stop() / (steps * len(plans))
+    del g
+    return ms
+
+
+def plan_signature(plan: api.Plan, resident: Resident, int_names: list[str], out_names: list[str],
+                   sample_words: int = 4096) -> tuple:
+    """Popcounts of every instance plus the first and last `sample_words` words
+    of each output stream of instances 0 and n-1."""
+    sig = []
+    n = len(resident.cases)
+    for i in range(n):
+        for name in int_names:
+            sig.append(plan.read_int(i, name))
+    for i in sorted({0, n - 1}):
+        nw = max(1, (resident.cases[i].bitlength + 63) // 64)
+        k = min(nw, sample_words)
+        for name in out_names:
+            sig.append(plan.download_output(i, name, k).tobytes())
+            sig.append(plan.download_output(i, name, k, offset=nw - k).tobytes())
+    return tuple(sig)
+
+
+def default_space(kind: str = "auto") -> list[dict]:
+    """Candidate schedules.  `kind` narrows the grid: "wide" for kernels with
+    many inputs (register-bound), "stateful" for add/advance chains, "small"
+    for batches of short streams."""
+    grid = {
+        "chunk_rows": [8, 16, 32],
+        "rows_per_step": [1, 2, 4, 8],
+        "occupancy": [0, 2, 4],
+        "waves_per_block": [4],
+    }
+    if kind == "wide":
+        grid = {"chunk_rows": [16, 32], "rows_per_step": [1, 2], "occupancy": [1, 2, 4],
+                "waves_per_block": [2, 4]}
+    elif kind == "stateful":
+        grid = {"chunk_rows": [4, 8, 16, 32, 64], "rows_per_step": [2, 4, 8], "occupancy": [0],
+                "waves_per_block": [1, 4]}
+    elif kind == "small":
+        grid = {"chunk_rows": [4, 8, 16], "rows_per_step": [2, 4], "occupancy": [0, 4],
+                "waves_per_block": [2, 4]}
+    keys = sorted(grid)
+    return [dict(zip(keys, v)) for v in itertools.product(*(grid[k] for k in keys))]
+
+
+@dataclass
+class Trial:
+    options: dict
+    ms: float
+    info: dict
+    compile_s: float
+
+
+@dataclass
+class TuneResult:
+    trials: list[Trial] = field(default_factory=list)
+    seconds: float = 0.0
+
+    @property
+    def best(self) -> Trial:
+        return min(self.trials, key=lambda t: t.ms)
+
+    def as_json(self) -> dict:
+        b = self.best
+        return {"candidates": len(self.trials), "tuning_s": round(self.seconds, 2),
+                "best": {"options": b.options, "ms": round(b.ms, 5)},
+                "worst_ms": round(max(t.ms for t in self.trials), 5)}
+
+
+def autotune(ctx: api.Context, program: api.Program, residents: list[Resident], space: list[dict],
+             int_names: list[str], out_names: list[str], steps: int = 10, warmup: int = 2,
+             base_options: dict | None = None, log=None) -> TuneResult:
+    """Measure every candidate of `space`: one plan per resident input set
+    (rotating through several sets keeps them out of the last-level cache),
+    all runs of a step captured together.  Raises if a candidate's outputs
+    differ from the first candidate's."""
+    t0 = time.perf_counter()
+    res = TuneResult()
+    shape = shape_case(residents[0].cases)
+    ref_sig = None
+    for cand in space:
+        opts = dict(base_options or {})
+        opts.update(cand)
+        tc = time.perf_counter()
+        plans = [api.Plan(ctx, program, shape, opts) for _ in residents]
+        compile_s = time.perf_counter() - tc
+        try:
+            for p, r in zip(plans, residents):
+                r.bind(p)
+                p.run()
+            ctx.sync()
+            sig = tuple(plan_signature(p, r, int_names, out_names) for p, r in zip(plans, residents))
+            if ref_sig is None:
+                ref_sig = sig
+            elif sig != ref_sig:
+                raise RuntimeError(f"schedule {cand} changed the outputs")
+            ms = time_runs(ctx, plans, steps, warmup)
+            res.trials.append(Trial(cand, ms, plans[0].info(), compile_s))
+        finally:
+            for p in plans:
+                p.close()
+        if log:
+            log(f"tune {cand}: {ms:.4f} ms")
+    res.seconds = time.perf_counter() - t0
+    return res
