@@ -107,7 +107,8 @@ void upload_stream(hipStream_t s, uint64_t* dst, const uint64_t* host, uint64_t 
 // Before a fused plan launch: for each of n_ctl control blocks of 4 words
 // (ticket, done, epoch, mismatches) clear ticket/done/mismatches and advance
 // the epoch; clear n_pop popcount accumulators.
-void launch_plan_prep(hipStream_t s, unsigned* ctl, uint32_t n_ctl, unsigned long long* pop, uint64_t n_pop);
+// Advances a fused plan's run counter (ctl[0]).
+void launch_plan_epoch(hipStream_t s, unsigned* ctl);
 // Copy-engine bandwidth probe kernel: dst[i] = src[i] over n words.
 void launch_copy_words(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t n);
 

@@ -254,9 +254,10 @@ public:
         size_t total_pop = 0;
         off_inst_ = a.take(sizeof(InstDesc) * n_inst);
         off_chunk_inst_ = n_inst > 1 ? a.take(4ull * n_chunks) : 0;
-        off_ctl_ = a.take(16ull * std::max<size_t>(1, K));
+        off_ctl_ = a.take(4ull * (4 + 4 * K));
         for (size_t k = 0; k < K; ++k) total_pop += P.kernels[k].pops.size() * n_inst;
-        off_pop_ = a.take(8ull * std::max<size_t>(1, total_pop));
+        pop_stride_ = total_pop * kPopSpread;
+        off_pop_ = a.take(8ull * std::max<size_t>(1, 2 * pop_stride_));
         size_t pop_cursor = 0;
         for (size_t k = 0; k < K; ++k) {
             const KernelSpec& ks = P.kernels[k];
@@ -267,14 +268,12 @@ public:
             r.nst = ks.n_state;
             r.in_tab = a.take(8ull * std::max<size_t>(1, r.nin * n_inst));
             r.out_tab = a.take(8ull * std::max<size_t>(1, r.nout * n_inst));
-            r.pop = off_pop_ + 8ull * pop_cursor;
+            r.pop = off_pop_ + 8ull * kPopSpread * pop_cursor;
             pop_cursor += (size_t)r.npop * n_inst;
             if (r.nst) {
                 r.st_u = a.take(8ull * n_chunks * r.nst);
                 r.st_t = a.take(8ull * n_chunks * r.nst);
-                r.flag = a.take(4ull * n_chunks);
                 r.partial = a.take(4ull * n_chunks * std::max(1, r.npop));
-                r.mism = a.take(4ull * n_chunks);
             }
         }
         total_pop_ = total_pop;
@@ -284,6 +283,10 @@ public:
             BSZ_HIP(hipMalloc(&dev_, a.bytes));
             BSZ_HIP(hipMemsetAsync(dev_, 0, a.bytes, s));
             dev_bytes_ = a.bytes;
+        } else {
+            // epoch 0 and both popcount banks clear for the new layout
+            BSZ_HIP(hipMemsetAsync(ptr(off_ctl_), 0, 4ull * (4 + 4 * K), s));
+            BSZ_HIP(hipMemsetAsync(ptr(off_pop_), 0, 8ull * std::max<size_t>(1, 2 * pop_stride_), s));
         }
         in_tab_h_.assign(K, {});
         out_tab_h_.assign(K, {});
@@ -316,42 +319,49 @@ public:
         tables_dirty_ = false;
     }
 
-    // prep + every kernel, asynchronously on s.
+    // Every kernel, asynchronously on s: the main launch, then, for kernels
+    // with state, the fix-up launch.  The run's last launch advances the epoch
+    // (a separate one-thread launch when the last kernel has no state).
     void launch(hipStream_t s) {
         const PlanSpec& P = *spec;
-        launch_plan_prep(s, (unsigned*)ptr(off_ctl_), (uint32_t)P.kernels.size(), (unsigned long long*)ptr(off_pop_),
-                         total_pop_);
         const int wpb = P.opt.waves_per_block;
         const uint32_t blocks = (n_chunks + wpb - 1) / wpb;
-        for (size_t k = 0; k < P.kernels.size(); ++k) {
+        const size_t K = P.kernels.size();
+        const int64_t before = launches_;
+        for (size_t k = 0; k < K; ++k) {
             const KRes& r = kres_[k];
             KArgs A{};
             A.in = (const uint64_t* const*)ptr(r.in_tab);
             A.out = (uint64_t* const*)ptr(r.out_tab);
             A.pop_acc = (unsigned long long*)ptr(r.pop);
-            A.pop_out = A.pop_acc;
             A.inst = (const InstDesc*)ptr(off_inst_);
             A.chunk_inst = n_inst > 1 ? (const uint32_t*)ptr(off_chunk_inst_) : nullptr;
             A.st_u = r.nst ? (unsigned long long*)ptr(r.st_u) : nullptr;
             A.st_t = r.nst ? (unsigned long long*)ptr(r.st_t) : nullptr;
-            A.st_flag = r.nst ? (unsigned*)ptr(r.flag) : nullptr;
             A.partial = r.nst ? (unsigned*)ptr(r.partial) : nullptr;
-            A.mismatch = r.nst ? (unsigned*)ptr(r.mism) : nullptr;
-            A.ctl = (unsigned*)ptr(off_ctl_ + 16ull * k);
+            A.ctl = (unsigned*)ptr(off_ctl_);
+            A.kstat = (unsigned*)ptr(off_ctl_ + 4ull * (4 + 4 * k));
+            A.pop_stride = pop_stride_;
             A.n_chunks = n_chunks;
             A.n_inst = n_inst;
             A.mode = 0;
+            A.last = 0;
             void* params[] = {&A};
             BSZ_HIP(hipModuleLaunchKernel(mod->fns[k], blocks, 1, 1, 64 * wpb, 1, 1, 0, s, params, nullptr));
             ++launches_;
             if (r.nst) {
-                // in-order recomputation of mis-speculated chunks (returns at once when there are none)
                 A.mode = 1;
-                BSZ_HIP(hipModuleLaunchKernel(mod->fns[k], 1, 1, 1, 64, 1, 1, 0, s, params, nullptr));
+                A.last = k + 1 == K ? 1u : 0u;
+                BSZ_HIP(hipModuleLaunchKernel(mod->fns[k], 1, 1, 1, kFixThreads, 1, 1, 4u * (kFixBitmapWords + 1), s,
+                                              params, nullptr));
                 ++launches_;
             }
         }
-        ++launches_;
+        if (K == 0 || !kres_[K - 1].nst) {
+            launch_plan_epoch(s, (unsigned*)ptr(off_ctl_));
+            ++launches_;
+        }
+        per_run_launches_ = launches_ - before;
     }
 
     void capture(hipStream_t s) {
@@ -367,15 +377,24 @@ public:
     void replay(hipStream_t s) {
         if (!graph_) throw RunError("plan has no captured graph");
         BSZ_HIP(hipGraphLaunch(graph_, s));
-        launches_ += 1 + (int64_t)spec->kernels.size();
+        launches_ += per_run_launches_;
     }
 
     // popcount of SegGraph pop `p` for instance `inst`, after a run (sync)
     void read_pops(hipStream_t s, std::vector<unsigned long long>& out) {
         out.assign(total_pop_, 0);
-        if (total_pop_)
-            BSZ_HIP(hipMemcpyAsync(out.data(), ptr(off_pop_), 8ull * total_pop_, hipMemcpyDeviceToHost, s));
+        if (!total_pop_) return;
+        uint32_t epoch = 0;
+        BSZ_HIP(hipMemcpyAsync(&epoch, ptr(off_ctl_), 4, hipMemcpyDeviceToHost, s));
         BSZ_HIP(hipStreamSynchronize(s));
+        if (epoch == 0) return;  // no run yet
+        const size_t bank = (epoch - 1) & 1u;
+        std::vector<unsigned long long> slots(pop_stride_);
+        BSZ_HIP(hipMemcpyAsync(slots.data(), ptr(off_pop_ + 8ull * bank * pop_stride_), 8ull * pop_stride_,
+                               hipMemcpyDeviceToHost, s));
+        BSZ_HIP(hipStreamSynchronize(s));
+        for (size_t q = 0; q < total_pop_; ++q)
+            for (int j = 0; j < kPopSpread; ++j) out[q] += slots[q * kPopSpread + j];
     }
     // index into read_pops() output of graph pop p for instance i
     size_t pop_index(size_t p, uint32_t inst) const {
@@ -391,7 +410,7 @@ public:
     // mismatching chunks seen by kernel k in the last run (sync)
     uint32_t mismatches(hipStream_t s, size_t k) {
         uint32_t v = 0;
-        BSZ_HIP(hipMemcpyAsync(&v, ptr(off_ctl_ + 16ull * k + 12), 4, hipMemcpyDeviceToHost, s));
+        BSZ_HIP(hipMemcpyAsync(&v, ptr(off_ctl_ + 4ull * (4 + 4 * k)), 4, hipMemcpyDeviceToHost, s));
         BSZ_HIP(hipStreamSynchronize(s));
         return v;
     }
@@ -402,11 +421,11 @@ private:
     struct KRes {
         size_t nin = 0, nout = 0;
         int npop = 0, nst = 0;
-        size_t in_tab = 0, out_tab = 0, pop = 0, st_u = 0, st_t = 0, flag = 0, partial = 0, mism = 0;
+        size_t in_tab = 0, out_tab = 0, pop = 0, st_u = 0, st_t = 0, partial = 0;
     };
     void* dev_ = nullptr;
     size_t dev_bytes_ = 0;
-    size_t off_inst_ = 0, off_chunk_inst_ = 0, off_ctl_ = 0, off_pop_ = 0, total_pop_ = 0;
+    size_t off_inst_ = 0, off_chunk_inst_ = 0, off_ctl_ = 0, off_pop_ = 0, total_pop_ = 0, pop_stride_ = 0;
     std::vector<KRes> kres_;
     std::vector<InstDesc> inst_h_;
     std::vector<uint32_t> chunk_inst_h_;
@@ -414,7 +433,7 @@ private:
     std::vector<std::vector<uint64_t*>> out_tab_h_;
     bool tables_dirty_ = true;
     hipGraphExec_t graph_ = nullptr;
-    int64_t launches_ = 0;
+    int64_t launches_ = 0, per_run_launches_ = 0;
 
     char* ptr(size_t off) const { return static_cast<char*>(dev_) + off; }
 };

@@ -41,10 +41,10 @@ typedef unsigned long long u64;
 typedef unsigned int u32;
 struct InstDesc { u64 rows; u64 lw; u64 lastmask; u32 chunk_begin; u32 n_chunks; };
 struct KArgs {
-  const u64* const* in; u64* const* out; u64* pop_acc; u64* pop_out;
+  const u64* const* in; u64* const* out; u64* pop_acc;
   const InstDesc* inst; const u32* chunk_inst;
-  u64* st_u; u64* st_t; u32* st_flag; u32* partial; u32* mismatch; u32* ctl;
-  u32 n_chunks; u32 n_inst; u32 mode;
+  u64* st_u; u64* st_t; u32* partial; u32* ctl; u32* kstat;
+  u64 pop_stride; u32 n_chunks; u32 n_inst; u32 mode; u32 last;
 };
 #define AGENT __HIP_MEMORY_SCOPE_AGENT
 // Binding tables are never written while a kernel runs: reading them through
@@ -114,10 +114,8 @@ __device__ __forceinline__ u32 aload32(u32* p) { return __hip_atomic_load(p, __A
 __device__ __forceinline__ u64 aload64(u64* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT); }
 __device__ __forceinline__ void astore32(u32* p, u32 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT); }
 __device__ __forceinline__ void astore64(u64* p, u64 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT); }
-// Wait until this wave's outstanding memory operations completed.  Cross-wave
-// data (states, flags, mismatch marks) is written and read with device-scope
-// atomics, which are coherent across XCDs without an L2 writeback, so a
-// completion wait before the flag store is all the ordering needed.
+// Wait until this wave's outstanding memory operations completed (the fix-up
+// wave re-reads end states it has just written).
 __device__ __forceinline__ void drain() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __builtin_amdgcn_s_waitcnt(0xF70);
@@ -223,45 +221,100 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
         while (occ > 1 && est_vgprs > 512 / occ) --occ;
     }
     ks.occupancy = occ;
-    s << "extern \"C\" __global__ __launch_bounds__(" << 64 * opt.waves_per_block
-      << ") __attribute__((amdgpu_waves_per_eu(" << occ << "))) void " << ks.name << "(KArgs A) {\n";
+    const int WPB = opt.waves_per_block;
+    const int LB = has_state ? std::max(64 * WPB, kFixThreads) : 64 * WPB;
+    ks.launch_bounds = LB;
+    const std::string NS = std::to_string(n_state) + "u";
+    const std::string SP = std::to_string(kPopSpread) + "u";
+    s << "extern \"C\" __global__ __launch_bounds__(" << LB << ") __attribute__((amdgpu_waves_per_eu(" << occ
+      << "))) void " << ks.name << "(KArgs A) {\n";
     s << "  const u32 lane = lane_id();\n";
     s << "  const u32 off8 = lane * 8u;\n";
-    s << "  const u32 tag = aload32(&A.ctl[2]) + 1u;\n";
-    // mode 0: the main launch, a queue of chunks; mode 1: one wave that
-    // recomputes, in order, the chunks whose speculative start state was wrong
     s << "  const bool fix = A.mode != 0u;\n";
-    s << "  u32 c = 0;\n";
-    s << "  if (!fix) {\n    if (lane == 0) c = __hip_atomic_fetch_add(&A.ctl[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
-    s << "    c = uni(c);\n  } else if (aload32(&A.ctl[3]) == 0u) {\n    return;\n  }\n";
-    s << "  u32 walk = 1;\n  bool prev_changed = false;\n";
-    s << "  for (;;) {\n";
-    s << "    if (!fix) {\n      if (c >= A.n_chunks) return;\n";
-    if (has_state) {
-        // find the next chunk to recompute
-        s << "    } else {\n      u32 nxt = 0xffffffffu;\n";
-        s << "      while (walk < A.n_chunks) {\n";
-        s << "        if (prev_changed) {\n";
-        s << "          const u32 pi = A.chunk_inst ? u32_at(A.chunk_inst, walk) : 0u;\n";
-        s << "          bool need = false;\n";
-        s << "          if (walk != inst_at(A.inst, pi).chunk_begin) {\n";
-        s << "            for (u32 i = lane; i < " << n_state << "u; i += 64u)\n";
-        s << "              if (aload64(&A.st_u[(u64)walk * " << n_state << "ull + i]) != aload64(&A.st_t[(u64)(walk - 1) * "
-          << n_state << "ull + i])) need = true;\n";
-        s << "            need = ballot(need) != 0ull;\n";
-        s << "          }\n";
-        s << "          if (need) { nxt = walk; break; }\n";
-        s << "          prev_changed = false;\n          ++walk;\n          continue;\n";
-        s << "        }\n";
-        s << "        const u32 x = walk + lane;\n";
-        s << "        const u64 hits = ballot(x < A.n_chunks && aload32(&A.mismatch[x]) != 0u);\n";
-        s << "        if (hits) { nxt = walk + (u32)__builtin_ctzll(hits); break; }\n";
-        s << "        walk += 64u;\n";
-        s << "      }\n";
-        s << "      if (nxt == 0xffffffffu) return;\n";
-        s << "      c = nxt;\n";
+    if (NPOP) {
+        // popcount slots of this run, and of the next one (cleared here)
+        s << "  const u32 ep = u32_at(A.ctl, 0u) & 1u;\n";
+        s << "  u64* const bank = A.pop_acc + (ep ? A.pop_stride : 0ull);\n";
     }
-    s << "    }\n";
+    s << "  u32 c = 0;\n";
+    if (has_state) {
+        // mode 1 (one workgroup): mark every chunk whose speculative start
+        // state differs from its predecessor's end state, in parallel
+        // dynamic LDS: only the fix-up launch asks for it, so the main launch's
+        // occupancy is not limited by the bitmap
+        s << "  extern __shared__ u32 s_dyn[];\n";
+        s << "  u32* const s_bits = s_dyn;\n  u32& s_any = s_dyn[" << kFixBitmapWords << "];\n";
+        s << "  u32 p_next = 1u, n_re = 0u, n_mis = 0u;\n  bool casc = false, use_bits = true;\n";
+        s << "  if (fix) {\n";
+        s << "    const bool fits = A.n_chunks <= 32u * " << kFixBitmapWords << "u;\n";
+        s << "    const u32 nbw = fits ? (A.n_chunks + 31u) / 32u : 0u;\n";
+        s << "    for (u32 i = threadIdx.x; i < nbw; i += blockDim.x) s_bits[i] = 0u;\n";
+        s << "    if (threadIdx.x == 0) s_any = 0u;\n";
+        s << "    __syncthreads();\n";
+        s << "    for (u32 x = 1u + threadIdx.x; fits && x < A.n_chunks; x += blockDim.x) {\n";
+        s << "      const u32 xi = A.chunk_inst ? A.chunk_inst[x] : 0u;\n";
+        s << "      if (x == A.inst[xi].chunk_begin) continue;\n";
+        s << "      bool bad = false;\n";
+        s << "      for (u32 i = 0; i < " << NS << "; ++i)\n";
+        s << "        bad |= A.st_u[(u64)x * " << NS << " + i] != A.st_t[(u64)(x - 1u) * " << NS << " + i];\n";
+        s << "      if (bad) { atomicOr(&s_bits[x >> 5], 1u << (x & 31u)); atomicAdd(&s_any, 1u); }\n";
+        s << "    }\n";
+        s << "    __syncthreads();\n";
+        s << "    if (threadIdx.x >= 64u) return;\n";
+        s << "    n_mis = s_any;\n";
+        s << "    use_bits = fits;\n";
+        s << "  } else {\n";
+    } else {
+        s << "  {\n";
+    }
+    // main launch: wave w of the grid owns chunk w
+    s << "    c = blockIdx.x * " << WPB << "u + uni(threadIdx.x >> 6);\n";
+    s << "    if (c >= A.n_chunks) return;\n";
+    if (NPOP) {
+        s << "    u64* const other = A.pop_acc + (ep ? 0ull : A.pop_stride);\n";
+        s << "    const u64 n_slots = (u64)A.n_inst * " << NPOP * kPopSpread << "ull;\n";
+        s << "    for (u64 i = (u64)c * 64ull + lane; i < n_slots; i += (u64)A.n_chunks * 64ull) other[i] = 0ull;\n";
+    }
+    s << "  }\n";
+    s << "  for (;;) {\n";
+    if (has_state) {
+        // mode 1, first wave: the next chunk to recompute, in order: a marked
+        // chunk, or the successor of a chunk whose end state changed
+        s << "    if (fix) {\n";
+        s << "      bool found = false;\n";
+        s << "      while (!found) {\n";
+        s << "        u32 cand = p_next;\n";
+        s << "        if (!casc && use_bits) {\n";
+        s << "          cand = 0xffffffffu;\n";
+        s << "          const u32 nbw = (A.n_chunks + 31u) >> 5;\n";
+        s << "          for (u32 wb = p_next >> 5; n_mis != 0u && wb < nbw; wb += 64u) {\n";
+        s << "            const u32 wi = wb + lane;\n";
+        s << "            u32 bits = wi < nbw ? s_bits[wi] : 0u;\n";
+        s << "            if (wi == (p_next >> 5)) bits &= ~0u << (p_next & 31u);\n";
+        s << "            const u64 hit = ballot(bits != 0u);\n";
+        s << "            if (hit) {\n";
+        s << "              const u32 l = (u32)__builtin_ctzll(hit);\n";
+        s << "              const u32 b = (u32)__builtin_amdgcn_readlane((int)bits, (int)l);\n";
+        s << "              cand = (wb + l) * 32u + (u32)__builtin_ctz(b);\n";
+        s << "              break;\n";
+        s << "            }\n";
+        s << "          }\n";
+        s << "        }\n";
+        s << "        if (cand >= A.n_chunks) goto fix_done;\n";
+        s << "        const u32 ci = A.chunk_inst ? u32_at(A.chunk_inst, cand) : 0u;\n";
+        s << "        bool need = false;\n";
+        s << "        if (cand != inst_at(A.inst, ci).chunk_begin) {\n";
+        s << "          for (u32 i = lane; i < " << NS << "; i += 64u)\n";
+        s << "            if (aload64(&A.st_u[(u64)cand * " << NS << " + i]) != aload64(&A.st_t[(u64)(cand - 1u) * " << NS
+          << " + i])) need = true;\n";
+        s << "          need = ballot(need) != 0ull;\n";
+        s << "        }\n";
+        s << "        p_next = cand + 1u;\n";
+        s << "        casc = false;\n";
+        s << "        if (need) { c = cand; found = true; }\n";
+        s << "      }\n";
+        s << "    }\n";
+    }
     s << "    const u32 ii = A.chunk_inst ? u32_at(A.chunk_inst, c) : 0u;\n";
     s << "    const InstDesc D = inst_at(A.inst, ii);\n";
     s << "    const u32 cc = c - D.chunk_begin;\n";
@@ -273,8 +326,14 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
     if (has_state) {
         s << "    const u64 warm = (!fix && cc > 0) ? (r_begin < " << WR << "ull ? r_begin : " << WR << "ull) : 0ull;\n";
         for (auto& kv : state_of)
-            s << "    u64 st" << kv.second << " = fix ? aload64(&A.st_t[(u64)(c - 1) * " << n_state << "ull + "
-              << kv.second << "ull]) : 0ull;\n";
+            s << "    u64 st" << kv.second << " = fix ? aload64(&A.st_t[(u64)(c - 1u) * " << NS << " + " << kv.second
+              << "u]) : 0ull;\n";
+        // the start state a recomputed chunk used
+        s << "    if (fix) {\n";
+        for (auto& kv : state_of)
+            s << "      if (lane == " << (kv.second % 64) << ") astore64(&A.st_u[(u64)c * " << NS << " + " << kv.second
+              << "u], st" << kv.second << ");\n";
+        s << "    }\n";
     } else {
         s << "    const u64 warm = 0ull;\n";
     }
@@ -326,9 +385,10 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
         s << "      if (ok" << u << ") {\n";
         if (has_state) {
             // the speculative state this chunk reached at its first row (U[c])
-            s << "        if (r + " << u << "ull == r_begin && !fix && lane == 0) {\n";
+            s << "        if (r + " << u << "ull == r_begin && !fix) {\n";
             for (auto& kv : state_of)
-                s << "          astore64(&A.st_u[(u64)c * " << n_state << "ull + " << kv.second << "ull], st" << kv.second << ");\n";
+                s << "          if (lane == " << (kv.second % 64) << ") A.st_u[(u64)c * " << NS << " + " << kv.second
+                  << "u] = st" << kv.second << ";\n";
             s << "        }\n";
         }
         s << "        const u64 m = word_mask(w" << u << ", D.lw, D.lastmask);\n";
@@ -409,63 +469,52 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
     }
     s << "    }\n";  // row loop
 
-    // popcount reduction and accumulation
+    // popcounts: one atomic per chunk into slot c mod kPopSpread
     for (int p = 0; p < NPOP; ++p) {
         s << "    {\n      const u32 tot = uni(wave_sum(pc" << p << "));\n";
         s << "      if (lane == 0) {\n";
-        s << "        u64* acc = &A.pop_acc[(u64)ii * " << NPOP << "u + " << p << "u];\n";
+        s << "        u64* acc = &bank[((u64)ii * " << NPOP << "u + " << p << "u) * " << SP << " + (c % " << SP << ")];\n";
         if (has_state) {
             s << "        u32* part = &A.partial[(u64)c * " << NPOP << "u + " << p << "u];\n";
             s << "        const u32 old = fix ? aload32(part) : 0u;\n";
-            s << "        __hip_atomic_fetch_add(acc, (u64)tot - (u64)old, __ATOMIC_RELAXED, AGENT);\n";
-            s << "        astore32(part, tot);\n";
+            s << "        if (tot != old) __hip_atomic_fetch_add(acc, (u64)tot - (u64)old, __ATOMIC_RELAXED, AGENT);\n";
+            s << "        if (fix) astore32(part, tot); else *part = tot;\n";
         } else {
             s << "        __hip_atomic_fetch_add(acc, (u64)tot, __ATOMIC_RELAXED, AGENT);\n";
         }
         s << "      }\n    }\n";
     }
-
     if (has_state) {
-        // publish T[c]; in fix mode also learn whether it changed
-        s << "    bool changed = false;\n";
-        s << "    {\n";
-        for (auto& kv : state_of) {
-            s << "      if (lane == " << (kv.second % 64) << ") {\n";
-            s << "        u64* tp = &A.st_t[(u64)c * " << n_state << "ull + " << kv.second << "ull];\n";
-            s << "        if (fix && aload64(tp) != st" << kv.second << ") changed = true;\n";
-            s << "        astore64(tp, st" << kv.second << ");\n";
-            s << "      }\n";
-        }
-        s << "      changed = ballot(changed) != 0ull;\n";
-        s << "      drain();\n";
-        s << "      if (lane == 0) astore32(&A.st_flag[c], tag);\n";
-        s << "    }\n";
-        // main mode: verify against the predecessor's published end state
+        // end state T[c]; a recomputed chunk also learns whether it changed
         s << "    if (!fix) {\n";
-        s << "      bool bad = false;\n";
-        s << "      if (cc > 0) {\n";
-        s << "        if (lane == 0) { while (aload32(&A.st_flag[c - 1]) != tag) __builtin_amdgcn_s_sleep(1); }\n";
-        s << "        for (u32 i = lane; i < " << n_state << "u; i += 64u)\n";
-        s << "          if (aload64(&A.st_u[(u64)c * " << n_state << "ull + i]) != aload64(&A.st_t[(u64)(c - 1) * "
-          << n_state << "ull + i])) bad = true;\n";
-        s << "        bad = ballot(bad) != 0ull;\n";
-        s << "      }\n";
-        s << "      if (lane == 0) {\n        astore32(&A.mismatch[c], bad ? 1u : 0u);\n";
-        s << "        if (bad) __hip_atomic_fetch_add(&A.ctl[3], 1u, __ATOMIC_RELAXED, AGENT);\n      }\n";
+        for (auto& kv : state_of)
+            s << "      if (lane == " << (kv.second % 64) << ") A.st_t[(u64)c * " << NS << " + " << kv.second << "u] = st"
+              << kv.second << ";\n";
+        s << "      return;\n";
         s << "    }\n";
+        s << "    bool changed = false;\n";
+        for (auto& kv : state_of) {
+            s << "    if (lane == " << (kv.second % 64) << ") {\n";
+            s << "      u64* tp = &A.st_t[(u64)c * " << NS << " + " << kv.second << "u];\n";
+            s << "      if (aload64(tp) != st" << kv.second << ") changed = true;\n";
+            s << "      astore64(tp, st" << kv.second << ");\n";
+            s << "    }\n";
+        }
+        s << "    casc = ballot(changed) != 0ull;\n";
+        s << "    ++n_re;\n";
+        s << "    drain();\n";
+        s << "  }\n";  // job loop
+        s << "fix_done:\n";
+        s << "  if (lane == 0) {\n";
+        s << "    astore32(&A.kstat[0], use_bits ? n_mis : n_re);\n";
+        s << "    astore32(&A.kstat[1], n_re);\n";
+        s << "    if (A.last) astore32(&A.ctl[0], aload32(&A.ctl[0]) + 1u);\n";
+        s << "  }\n";
+    } else {
+        s << "    return;\n";
+        s << "  }\n";  // job loop (one pass)
     }
-
-    // next job
-    s << "    if (!fix) {\n";
-    s << "      if (lane == 0) c = __hip_atomic_fetch_add(&A.ctl[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
-    s << "      c = uni(c);\n";
-    if (has_state) s << "    } else {\n      prev_changed = changed;\n      walk = c + 1u;\n";
-    s << "    }\n";
-    s << "  }\n";  // job loop
     s << "}\n";
-    // The control block and the popcount accumulators are reset by the plan's
-    // prep kernel before every launch, so a wave that starts after the last
-    // chunk finished only finds the ticket counter past n_chunks.
     ks.source = s.str();
 }
 

@@ -15,16 +15,23 @@
 //   * those scalars (shift neighbours and carries) are the kernel's state.  A
 //     chunk other than the first starts `warmup_rows` rows early with a zero
 //     state (speculation), records the state it reaches at its first row (U)
-//     and publishes the state at its end (T).  Chunk c was exact iff
-//     U[c] == T[c-1]; each chunk checks that against its predecessor, and a
-//     second launch of the same kernel (one wave, mode 1) recomputes, in
-//     order, every chunk whose check failed from its predecessor's true end
-//     state.  The result is exact for every input; speculation only decides
-//     how much work is redone;
+//     and the state at its end (T).  Chunk c was exact iff U[c] == T[c-1].
+//     The main launch has no cross-wave dependency at all (no tickets, no
+//     waits, no flags): wave w of the grid owns chunk w.  A second launch of
+//     the same kernel (one workgroup, mode 1) compares U[c] with T[c-1] for
+//     every chunk in parallel, marks the mismatches in an LDS bitmap, and one
+//     wave recomputes them in order from the predecessor's true end state,
+//     following a changed end state into the next chunk.  The result is exact
+//     for every input; speculation only decides how much work is redone;
 //   * an advance of a value computed by an earlier kernel (any k) and `>>` are
 //     direct loads at the shifted word offset, needing no state;
 //   * popcounts accumulate per lane (v_bcnt), are reduced per wave and added
-//     atomically per chunk.
+//     with one atomic per chunk into one of kPopSpread slots (chunk mod
+//     kPopSpread), so no address sees more than 1/kPopSpread of a stream's
+//     chunks; the host sums the slots.  Slots live in two banks: a run adds
+//     into bank (epoch & 1) and clears the other for the next run, and the
+//     run's last launch advances the epoch, so no separate clearing launch is
+//     needed.
 #pragma once
 
 #include <cstdint>
@@ -79,6 +86,7 @@ struct KernelSpec {
     int rows_per_step = 1;           // as generated (capped by register budget)
     int chunk_rows = 16;             // as generated (a multiple of rows_per_step)
     int occupancy = 4;               // waves per SIMD the register budget was set for
+    int launch_bounds = 256;         // largest workgroup the kernel accepts
     std::string source;              // generated HIP
 };
 
@@ -104,22 +112,32 @@ struct InstDesc {
     uint32_t n_chunks;
 };
 
+// Popcount accumulator slots per (instance, popcount).
+constexpr int kPopSpread = 16;
+// LDS bitmap of the fix-up launch: chunks beyond it are checked one by one.
+constexpr uint32_t kFixBitmapWords = 8192;
+// Threads of the fix-up launch.
+constexpr int kFixThreads = 256;
+
+// ctl block of a plan: [0] epoch (runs completed); per kernel k, at
+// 4 + 4k: [0] chunks whose speculation failed in the last run, [1] chunks
+// recomputed.
 struct KArgs {
     const uint64_t* const* in;        // [inst][n_in]
     uint64_t* const* out;             // [inst][n_out]
-    unsigned long long* pop_acc;      // [inst][n_pop_k]   zeroed again by the last wave
-    unsigned long long* pop_out;      // [inst][n_pop_k]
+    unsigned long long* pop_acc;      // bank 0: [inst][n_pop_k][kPopSpread]; bank 1 at + pop_stride
     const InstDesc* inst;
     const uint32_t* chunk_inst;       // [n_chunks] instance of each chunk
     unsigned long long* st_u;         // [n_chunks][n_state]
     unsigned long long* st_t;         // [n_chunks][n_state]
-    unsigned* st_flag;                // [n_chunks] epoch tag of a published T
-    unsigned* partial;                // [n_chunks][n_pop_k]
-    unsigned* mismatch;               // [n_chunks]
-    unsigned* ctl;                    // ticket, done, epoch, mismatches, recomputed
+    unsigned* partial;                // [n_chunks][n_pop_k] (stateful kernels)
+    unsigned* ctl;                    // the plan's ctl block
+    unsigned* kstat;                  // this kernel's stats in it
+    uint64_t pop_stride;              // words between the two banks
     uint32_t n_chunks;
     uint32_t n_inst;
-    uint32_t mode;                    // 0 main launch, 1 in-order fix-up (one wave)
+    uint32_t mode;                    // 0 main launch, 1 fix-up (one workgroup)
+    uint32_t last;                    // 1: this launch ends the run (advances the epoch)
 };
 
 }  // namespace bsz

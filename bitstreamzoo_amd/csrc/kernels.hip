@@ -248,15 +248,10 @@ __global__ __launch_bounds__(256) void k_tail(uint64_t* dst, uint64_t k, uint64_
     if (blockIdx.x == 0 && threadIdx.x == 0 && lw >= 1 && lw - 1 < k) dst[lw - 1] &= lastmask;
 }
 
-__global__ __launch_bounds__(256) void k_plan_prep(unsigned* ctl, uint32_t n_ctl, unsigned long long* pop, uint64_t n_pop) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_ctl) {
-        ctl[4 * i + 0] = 0u;
-        ctl[4 * i + 1] = 0u;
-        ctl[4 * i + 2] += 1u;
-        ctl[4 * i + 3] = 0u;
-    }
-    if (i < n_pop) pop[i] = 0ull;
+// Ends a fused plan run whose last kernel has no fix-up launch: the run
+// counter selects the popcount bank the next run adds into.
+__global__ __launch_bounds__(64) void k_plan_epoch(unsigned* ctl) {
+    if (threadIdx.x == 0) ctl[0] += 1u;
 }
 
 __global__ __launch_bounds__(256) void k_copy_words(uint64_t* __restrict__ dst, const uint64_t* __restrict__ src,
@@ -350,10 +345,8 @@ void upload_stream(hipStream_t s, uint64_t* dst, const uint64_t* host, uint64_t 
     check_launch();
 }
 
-void launch_plan_prep(hipStream_t s, unsigned* ctl, uint32_t n_ctl, unsigned long long* pop, uint64_t n_pop) {
-    const uint64_t n = std::max<uint64_t>(n_ctl, n_pop);
-    if (!n) return;
-    k_plan_prep<<<blocks_for(n, 256), 256, 0, s>>>(ctl, n_ctl, pop, n_pop);
+void launch_plan_epoch(hipStream_t s, unsigned* ctl) {
+    k_plan_epoch<<<1, 64, 0, s>>>(ctl);
     check_launch();
 }
 

@@ -7,6 +7,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 HERE = os.path.dirname(os.path.abspath(__file__))
+# Parallel test workers each running the OpenMP CPU executors with every core
+# oversubscribe the machine many times over; give each worker one thread.
+if os.environ.get("PYTEST_XDIST_WORKER"):
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
 if HERE not in sys.path:
     sys.path.insert(0, HERE)
 
