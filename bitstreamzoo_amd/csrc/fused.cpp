@@ -256,7 +256,11 @@ public:
         off_chunk_inst_ = n_inst > 1 ? a.take(4ull * n_chunks) : 0;
         off_ctl_ = a.take(4ull * (4 + 4 * K));
         for (size_t k = 0; k < K; ++k) total_pop += P.kernels[k].pops.size() * n_inst;
-        pop_stride_ = total_pop * kPopSpread;
+        uint32_t max_chunks = 1;
+        for (const InstDesc& d : inst_h_) max_chunks = std::max(max_chunks, d.n_chunks);
+        spread_log_ = 0;
+        while (spread_log_ < (uint32_t)kMaxPopSpreadLog && (4u << spread_log_) < max_chunks) ++spread_log_;
+        pop_stride_ = total_pop << spread_log_;
         off_pop_ = a.take(8ull * std::max<size_t>(1, 2 * pop_stride_));
         size_t pop_cursor = 0;
         for (size_t k = 0; k < K; ++k) {
@@ -268,7 +272,7 @@ public:
             r.nst = ks.n_state;
             r.in_tab = a.take(8ull * std::max<size_t>(1, r.nin * n_inst));
             r.out_tab = a.take(8ull * std::max<size_t>(1, r.nout * n_inst));
-            r.pop = off_pop_ + 8ull * kPopSpread * pop_cursor;
+            r.pop = off_pop_ + (8ull << spread_log_) * pop_cursor;
             pop_cursor += (size_t)r.npop * n_inst;
             if (r.nst) {
                 r.st_u = a.take(8ull * n_chunks * r.nst);
@@ -346,6 +350,7 @@ public:
             A.n_inst = n_inst;
             A.mode = 0;
             A.last = 0;
+            A.spread_log = spread_log_;
             void* params[] = {&A};
             BSZ_HIP(hipModuleLaunchKernel(mod->fns[k], blocks, 1, 1, 64 * wpb, 1, 1, 0, s, params, nullptr));
             ++launches_;
@@ -393,8 +398,9 @@ public:
         BSZ_HIP(hipMemcpyAsync(slots.data(), ptr(off_pop_ + 8ull * bank * pop_stride_), 8ull * pop_stride_,
                                hipMemcpyDeviceToHost, s));
         BSZ_HIP(hipStreamSynchronize(s));
+        const size_t S = size_t(1) << spread_log_;
         for (size_t q = 0; q < total_pop_; ++q)
-            for (int j = 0; j < kPopSpread; ++j) out[q] += slots[q * kPopSpread + j];
+            for (size_t j = 0; j < S; ++j) out[q] += slots[q * S + j];
     }
     // index into read_pops() output of graph pop p for instance i
     size_t pop_index(size_t p, uint32_t inst) const {
@@ -426,6 +432,7 @@ private:
     void* dev_ = nullptr;
     size_t dev_bytes_ = 0;
     size_t off_inst_ = 0, off_chunk_inst_ = 0, off_ctl_ = 0, off_pop_ = 0, total_pop_ = 0, pop_stride_ = 0;
+    uint32_t spread_log_ = 0;
     std::vector<KRes> kres_;
     std::vector<InstDesc> inst_h_;
     std::vector<uint32_t> chunk_inst_h_;

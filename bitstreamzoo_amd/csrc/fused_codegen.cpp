@@ -44,7 +44,7 @@ struct KArgs {
   const u64* const* in; u64* const* out; u64* pop_acc;
   const InstDesc* inst; const u32* chunk_inst;
   u64* st_u; u64* st_t; u32* partial; u32* ctl; u32* kstat;
-  u64 pop_stride; u32 n_chunks; u32 n_inst; u32 mode; u32 last;
+  u64 pop_stride; u32 n_chunks; u32 n_inst; u32 mode; u32 last; u32 spread_log; u32 pad_;
 };
 #define AGENT __HIP_MEMORY_SCOPE_AGENT
 // Binding tables are never written while a kernel runs: reading them through
@@ -193,7 +193,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
             if (part) off_words.insert({x.a, q + 1});
         }
     }
-    // The chunk length is the plan's, shared by all its kernels; U divides it.
+    // The chunk length is the plan's, shared by all its kernels.
     const int loads_per_row = (int)(plain_inputs.size() + off_words.size());
     const int CR = std::max(1, opt.chunk_rows);
     // With an explicit occupancy the prefetch may fill that register budget.
@@ -202,10 +202,10 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
     const int load_budget =
         opt.occupancy > 0 ? std::max(0, 512 / opt.occupancy - 24 - NPOP_ - live_) : 96;
     const int u_cap = std::max(1, std::min(opt.rows_per_step, load_budget / std::max(1, 2 * loads_per_row)));
-    int U = 1;
-    for (int d = 1; d <= std::min(u_cap, CR); ++d)
-        if (CR % d == 0) U = d;
     const int WR = has_state ? std::max(1, opt.warmup_rows) : 0;
+    // A step may cover a whole chunk including its warm-up rows, so that a
+    // short chunk is one round of loads; rows past the chunk are skipped.
+    const int U = std::min(u_cap, CR + WR);
     ks.rows_per_step = U;
     ks.chunk_rows = CR;
 
@@ -225,7 +225,6 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
     const int LB = has_state ? std::max(64 * WPB, kFixThreads) : 64 * WPB;
     ks.launch_bounds = LB;
     const std::string NS = std::to_string(n_state) + "u";
-    const std::string SP = std::to_string(kPopSpread) + "u";
     s << "extern \"C\" __global__ __launch_bounds__(" << LB << ") __attribute__((amdgpu_waves_per_eu(" << occ
       << "))) void " << ks.name << "(KArgs A) {\n";
     s << "  const u32 lane = lane_id();\n";
@@ -272,7 +271,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
     s << "    if (c >= A.n_chunks) return;\n";
     if (NPOP) {
         s << "    u64* const other = A.pop_acc + (ep ? 0ull : A.pop_stride);\n";
-        s << "    const u64 n_slots = (u64)A.n_inst * " << NPOP * kPopSpread << "ull;\n";
+        s << "    const u64 n_slots = ((u64)A.n_inst * " << NPOP << "ull) << A.spread_log;\n";
         s << "    for (u64 i = (u64)c * 64ull + lane; i < n_slots; i += (u64)A.n_chunks * 64ull) other[i] = 0ull;\n";
     }
     s << "  }\n";
@@ -469,11 +468,12 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
     }
     s << "    }\n";  // row loop
 
-    // popcounts: one atomic per chunk into slot c mod kPopSpread
+    // popcounts: one atomic per chunk into slot c mod 2^spread_log
     for (int p = 0; p < NPOP; ++p) {
         s << "    {\n      const u32 tot = uni(wave_sum(pc" << p << "));\n";
         s << "      if (lane == 0) {\n";
-        s << "        u64* acc = &bank[((u64)ii * " << NPOP << "u + " << p << "u) * " << SP << " + (c % " << SP << ")];\n";
+        s << "        u64* acc = &bank[(((u64)ii * " << NPOP << "u + " << p
+          << "u) << A.spread_log) + (c & ((1u << A.spread_log) - 1u))];\n";
         if (has_state) {
             s << "        u32* part = &A.partial[(u64)c * " << NPOP << "u + " << p << "u];\n";
             s << "        const u32 old = fix ? aload32(part) : 0u;\n";

@@ -26,9 +26,10 @@
 //   * an advance of a value computed by an earlier kernel (any k) and `>>` are
 //     direct loads at the shifted word offset, needing no state;
 //   * popcounts accumulate per lane (v_bcnt), are reduced per wave and added
-//     with one atomic per chunk into one of kPopSpread slots (chunk mod
-//     kPopSpread), so no address sees more than 1/kPopSpread of a stream's
-//     chunks; the host sums the slots.  Slots live in two banks: a run adds
+//     with one atomic per chunk into one of 2^spread_log slots (chunk mod
+//     2^spread_log, up to kMaxPopSpread slots, about one per 4 chunks of the
+//     longest instance), so no address serializes more than a few atomics;
+//     the host sums the slots.  Slots live in two banks: a run adds
 //     into bank (epoch & 1) and clears the other for the next run, and the
 //     run's last launch advances the epoch, so no separate clearing launch is
 //     needed.
@@ -84,7 +85,7 @@ struct KernelSpec {
     std::vector<int32_t> pops;       // index into SegGraph::pops accumulated here
     int n_state = 0;                 // 64-bit state words
     int rows_per_step = 1;           // as generated (capped by register budget)
-    int chunk_rows = 16;             // as generated (a multiple of rows_per_step)
+    int chunk_rows = 16;             // as generated
     int occupancy = 4;               // waves per SIMD the register budget was set for
     int launch_bounds = 256;         // largest workgroup the kernel accepts
     std::string source;              // generated HIP
@@ -112,8 +113,8 @@ struct InstDesc {
     uint32_t n_chunks;
 };
 
-// Popcount accumulator slots per (instance, popcount).
-constexpr int kPopSpread = 16;
+// Most popcount accumulator slots per (instance, popcount).
+constexpr int kMaxPopSpreadLog = 8;
 // LDS bitmap of the fix-up launch: chunks beyond it are checked one by one.
 constexpr uint32_t kFixBitmapWords = 8192;
 // Threads of the fix-up launch.
@@ -125,7 +126,7 @@ constexpr int kFixThreads = 256;
 struct KArgs {
     const uint64_t* const* in;        // [inst][n_in]
     uint64_t* const* out;             // [inst][n_out]
-    unsigned long long* pop_acc;      // bank 0: [inst][n_pop_k][kPopSpread]; bank 1 at + pop_stride
+    unsigned long long* pop_acc;      // bank 0: [inst][n_pop_k][2^spread_log]; bank 1 at + pop_stride
     const InstDesc* inst;
     const uint32_t* chunk_inst;       // [n_chunks] instance of each chunk
     unsigned long long* st_u;         // [n_chunks][n_state]
@@ -138,6 +139,8 @@ struct KArgs {
     uint32_t n_inst;
     uint32_t mode;                    // 0 main launch, 1 fix-up (one workgroup)
     uint32_t last;                    // 1: this launch ends the run (advances the epoch)
+    uint32_t spread_log;              // log2 of the popcount slots per (instance, popcount)
+    uint32_t pad_;
 };
 
 }  // namespace bsz
