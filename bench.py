@@ -32,6 +32,20 @@ from bitstreamzoo_amd import api, tuning, workloads  # noqa: E402
 from bitstreamzoo_amd.bsdata import BsCase  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "adder32_fused_hbm_traffic.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the fused adder kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 METRIC = "stream Gbit-ops/s + % HBM roofline; learned vs autotuned schedule time"
 
 
@@ -317,7 +331,8 @@ def main():
             "executor": "fused", "bitlength": w.cases[0].bitlength if args.workload != "shift_and" else None,
             "kernels": len(r["plan"]["kernels"]), "schedule": r["options"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": profiled_traffic() if args.workload == "adder32" else None,
                      "measured_copy_GBps": round(copy_gbps, 1)},
     }
     if r.get("tune"):
