@@ -14,7 +14,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <atomic>
 #include <mutex>
+#include <set>
+#include <thread>
 #include <sstream>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -129,71 +132,138 @@ void mkdirs(const std::string& path) {
     mkdir(path.c_str(), 0755);
 }
 
+// The entry points of a plan's kernels (code objects owned by the Jit).
 struct Module {
-    hipModule_t mod = nullptr;
     std::vector<hipFunction_t> fns;
-    ~Module() {
-        if (mod) (void)hipModuleUnload(mod);
-    }
 };
 using ModuleP = std::shared_ptr<Module>;
 
-// Code objects cached in memory by source hash and on disk, so a plan compiled
-// once (in this process or an earlier one) is loaded without hipRTC.
+// Code objects cached in memory and on disk by the hash of their unit, so a
+// kernel compiled once (by any plan, in this process or an earlier one) is
+// loaded without hipRTC.  Missing units of a plan are compiled in parallel.
 class Jit {
 public:
     explicit Jit(std::string dir) : dir_(cache_root(dir)) {}
+    ~Jit() {
+        for (auto& kv : loaded_)
+            if (kv.second.first) (void)hipModuleUnload(kv.second.first);
+    }
 
-    std::vector<char> code_for(const std::string& src, double* compile_ms) {
+    // Compiles (or finds cached) every kernel unit of `p`; no device needed.
+    void compile_all(const PlanSpec& p, double* compile_ms) {
         const std::string arch = default_arch();
-        const uint64_t h = source_hash(src + "|" + arch);
-        char name[64];
-        std::snprintf(name, sizeof name, "/%016llx.co", (unsigned long long)h);
-        const std::string path = dir_ + name;
-        if (const char* dump = std::getenv("BSZ_DUMP_DIR")) {
-            mkdirs(dump);
-            std::ofstream f(std::string(dump) + std::string(name).substr(0, 17) + ".hip");
-            f << src;
+        std::vector<const KernelSpec*> todo;
+        std::set<uint64_t> seen;
+        for (const KernelSpec& k : p.kernels) {
+            const uint64_t h = unit_hash(k.unit, arch);
+            dump(k.unit, h);
+            if (seen.count(h) || have_code(h)) continue;
+            seen.insert(h);
+            todo.push_back(&k);
         }
-        {
-            std::ifstream f(path, std::ios::binary);
-            if (f) {
-                std::vector<char> code((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-                if (!code.empty()) return code;
-            }
-        }
+        if (todo.empty()) return;
         auto t0 = std::chrono::steady_clock::now();
-        std::vector<char> code = compile_module(src, arch, nullptr);
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t nthr = std::min<size_t>(todo.size(), std::min(16u, hw));
+        std::atomic<size_t> next{0};
+        std::mutex err_mu;
+        std::string err;
+        auto work = [&] {
+            for (;;) {
+                const size_t i = next.fetch_add(1);
+                if (i >= todo.size()) return;
+                try {
+                    std::vector<char> code = compile_module(todo[i]->unit, arch, nullptr);
+                    store(unit_hash(todo[i]->unit, arch), code);
+                } catch (const std::exception& e) {
+                    std::lock_guard<std::mutex> g(err_mu);
+                    if (err.empty()) err = e.what();
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nthr; ++t) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
         if (compile_ms) *compile_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        mkdirs(dir_);
-        const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
-        {
-            std::ofstream f(tmp, std::ios::binary);
-            f.write(code.data(), (std::streamsize)code.size());
-        }
-        std::rename(tmp.c_str(), path.c_str());
-        return code;
+        if (!err.empty()) throw RunError(err);
     }
 
     ModuleP load(const PlanSpec& p, double* compile_ms) {
-        const uint64_t h = source_hash(p.module_source);
-        auto it = loaded_.find(h);
-        if (it != loaded_.end()) return it->second;
-        std::vector<char> code = code_for(p.module_source, compile_ms);
+        compile_all(p, compile_ms);
+        const std::string arch = default_arch();
         auto m = std::make_shared<Module>();
-        BSZ_HIP(hipModuleLoadData(&m->mod, code.data()));
         for (const KernelSpec& k : p.kernels) {
-            hipFunction_t f;
-            BSZ_HIP(hipModuleGetFunction(&f, m->mod, k.name.c_str()));
-            m->fns.push_back(f);
+            const uint64_t h = unit_hash(k.unit, arch);
+            auto it = loaded_.find(h);
+            if (it == loaded_.end()) {
+                std::vector<char> code = read_code(h);
+                if (code.empty()) throw RunError("internal: kernel code object missing after compilation");
+                hipModule_t mod;
+                BSZ_HIP(hipModuleLoadData(&mod, code.data()));
+                hipFunction_t f;
+                BSZ_HIP(hipModuleGetFunction(&f, mod, "bsz_kernel"));
+                it = loaded_.emplace(h, std::make_pair(mod, f)).first;
+            }
+            m->fns.push_back(it->second.second);
         }
-        loaded_[h] = m;
         return m;
     }
 
 private:
     std::string dir_;
-    std::unordered_map<uint64_t, ModuleP> loaded_;
+    std::mutex mu_;
+    std::unordered_map<uint64_t, std::vector<char>> codes_;                        // compiled this process
+    std::unordered_map<uint64_t, std::pair<hipModule_t, hipFunction_t>> loaded_;    // on the device
+
+    static uint64_t unit_hash(const std::string& unit, const std::string& arch) { return source_hash(unit + "|" + arch); }
+    std::string path_of(uint64_t h) const {
+        char name[64];
+        std::snprintf(name, sizeof name, "/%016llx.co", (unsigned long long)h);
+        return dir_ + name;
+    }
+    void dump(const std::string& unit, uint64_t h) const {
+        if (const char* d = std::getenv("BSZ_DUMP_DIR")) {
+            mkdirs(d);
+            char name[64];
+            std::snprintf(name, sizeof name, "/%016llx.hip", (unsigned long long)h);
+            std::ofstream f(std::string(d) + name);
+            f << unit;
+        }
+    }
+    bool have_code(uint64_t h) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            if (codes_.count(h)) return true;
+        }
+        std::ifstream f(path_of(h), std::ios::binary);
+        return (bool)f && f.peek() != std::ifstream::traits_type::eof();
+    }
+    std::vector<char> read_code(uint64_t h) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto it = codes_.find(h);
+            if (it != codes_.end()) return it->second;
+        }
+        std::ifstream f(path_of(h), std::ios::binary);
+        if (!f) return {};
+        return std::vector<char>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    }
+    void store(uint64_t h, const std::vector<char>& code) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            codes_[h] = code;
+        }
+        mkdirs(dir_);
+        const std::string path = path_of(h);
+        std::ostringstream tmpn;
+        tmpn << path << ".tmp" << (long)getpid() << "_" << std::this_thread::get_id();
+        {
+            std::ofstream f(tmpn.str(), std::ios::binary);
+            f.write(code.data(), (std::streamsize)code.size());
+        }
+        std::rename(tmpn.str().c_str(), path.c_str());
+    }
 };
 
 // ════════════════════════════════════════════════════════════════════════
@@ -923,10 +993,7 @@ std::unique_ptr<FusedPlan> FusedPlan::record(hipStream_t s, const Program& prog,
         I.ints.push_back(x);
     }
     I.spec = std::make_shared<PlanSpec>(build_plan(g, to_plan_options(opt)));
-    if (compile && !I.spec->kernels.empty()) {
-        std::vector<char> code = I.rec->jit().code_for(I.spec->module_source, &I.rec->compile_ms());
-        (void)code;
-    }
+    if (compile && !I.spec->kernels.empty()) I.rec->jit().compile_all(*I.spec, &I.rec->compile_ms());
     return fp;
 }
 

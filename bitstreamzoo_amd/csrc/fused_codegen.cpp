@@ -226,7 +226,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
     ks.launch_bounds = LB;
     const std::string NS = std::to_string(n_state) + "u";
     s << "extern \"C\" __global__ __launch_bounds__(" << LB << ") __attribute__((amdgpu_waves_per_eu(" << occ
-      << "))) void " << ks.name << "(KArgs A) {\n";
+      << "))) void bsz_kernel(KArgs A) {\n";
     s << "  const u32 lane = lane_id();\n";
     s << "  const u32 off8 = lane * 8u;\n";
     s << "  const bool fix = A.mode != 0u;\n";
@@ -692,11 +692,15 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt) {
     P.out_node_kernel.clear();
     for (int32_t o : g.outputs) P.out_node_kernel.push_back(out_kernel[o]);
 
+    // One compilation unit per kernel (prelude + kernel), so identical kernels
+    // of different plans share a cached code object and units compile in
+    // parallel; module_source lists them all for inspection.
     std::ostringstream mod;
     mod << kKernelPrelude;
     for (KernelSpec& ks : P.kernels) {
         gen_kernel(g, opt, ks);
-        mod << ks.source;
+        ks.unit = std::string(kKernelPrelude) + ks.source;
+        mod << "// ---- " << ks.name << " ----\n" << ks.source;
     }
     P.module_source = mod.str();
     return P;

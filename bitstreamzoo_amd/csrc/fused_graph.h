@@ -88,7 +88,8 @@ struct KernelSpec {
     int chunk_rows = 16;             // as generated
     int occupancy = 4;               // waves per SIMD the register budget was set for
     int launch_bounds = 256;         // largest workgroup the kernel accepts
-    std::string source;              // generated HIP
+    std::string source;              // generated HIP (entry point bsz_kernel)
+    std::string unit;                // self-contained compilation unit: prelude + source
 };
 
 struct PlanSpec {
@@ -97,7 +98,7 @@ struct PlanSpec {
     std::vector<KernelSpec> kernels;
     int32_t n_slots = 0;             // intermediate buffers (after interval reuse)
     std::vector<int32_t> out_node_kernel;  // per g.outputs entry: kernel storing it, -1 if it is a source
-    std::string module_source;       // all kernels, one hipRTC program
+    std::string module_source;       // all kernels, for inspection (each compiles as its own unit)
 };
 
 // Dead-code elimination, partitioning, slot assignment and code generation.

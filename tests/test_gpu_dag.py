@@ -1,0 +1,59 @@
+"""Fused plans of generated random regex-style DAGs (many kernels, slots,
+chained adds, MatchStar / ScanThru, advances) against the oracle, under
+several schedules, and as a batch of texts of uneven length."""
+
+import pytest
+
+from bitstreamzoo_amd import api, dag_gen, tuning
+from bitstreamzoo_amd.bsdata import words_to_int
+from cpu_models import Oracle
+from hip_models import ctx
+from runner import run_model
+
+pytestmark = pytest.mark.gpu
+
+SCHEDULES = [{}, {"chunk_rows": 2, "rows_per_step": 3, "max_live": 24, "max_ops": 300},
+             {"chunk_rows": 8, "rows_per_step": 1, "max_live": 160, "occupancy": 1}]
+
+
+def oracle_outputs(p, case):
+    names = Oracle(p.source).output_names()
+    o, _ = run_model(Oracle, p.source, case, Oracle.BEST, stream_names=names[0], int_names=names[1])
+    return o
+
+
+@pytest.mark.parametrize("seed,n_bytes", [(0, 5000), (1, 64 * 64 * 9 + 5), (2, 200_003)])
+@pytest.mark.parametrize("sched", range(len(SCHEDULES)))
+def test_dag_plan_matches_oracle(seed, n_bytes, sched):
+    p = dag_gen.generate(1200, seed=seed)
+    case = dag_gen.text_case(n_bytes, seed=seed)
+    exp = oracle_outputs(p, case)
+    c = ctx()
+    res = tuning.Resident(c, [case])
+    plan = api.Plan(c, api.Program(p.source), tuning.shape_case([case]), SCHEDULES[sched])
+    res.bind(plan)
+    plan.run()
+    nw = (n_bytes + 63) // 64
+    for name in p.outputs:
+        got = words_to_int(plan.download_output(0, name, nw))
+        assert got == exp[name], f"{name} (kernels={len(plan.info()['kernels'])})"
+    plan.close()
+    res.free()
+
+
+def test_dag_batch_of_uneven_texts():
+    p = dag_gen.generate(800, seed=7)
+    cases = [dag_gen.text_case(n, seed=10 + i) for i, n in enumerate((100, 4096, 64 * 64 * 3 + 1, 70_000))]
+    c = ctx()
+    res = tuning.Resident(c, cases)
+    plan = api.Plan(c, api.Program(p.source), tuning.shape_case(cases), {"chunk_rows": 4})
+    res.bind(plan)
+    plan.capture()
+    plan.replay(3)
+    for i, case in enumerate(cases):
+        exp = oracle_outputs(p, case)
+        nw = (case.bitlength + 63) // 64
+        for name in p.outputs[:16]:
+            assert words_to_int(plan.download_output(i, name, nw)) == exp[name], (i, name)
+    plan.close()
+    res.free()
