@@ -18,6 +18,7 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "bsz_hip.h")
 
 u64p = C.POINTER(C.c_uint64)
 i64p = C.POINTER(C.c_int64)
+f32p = C.POINTER(C.c_float)
 vp = C.c_void_p
 
 # name -> (restype, argtypes)
@@ -87,6 +88,16 @@ _SIGS = {
     "bsz_capture_end": (C.c_int, [vp, C.POINTER(vp)]),
     "bsz_graph_launch": (C.c_int, [vp, vp, C.c_int]),
     "bsz_graph_destroy": (C.c_int, [vp]),
+    "bsz_mlp_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),
+    "bsz_mlp_free": (C.c_int, [vp]),
+    "bsz_mlp_n_params": (C.c_int64, [vp]),
+    "bsz_mlp_get_params": (C.c_int, [vp, f32p]),
+    "bsz_mlp_set_params": (C.c_int, [vp, f32p]),
+    "bsz_mlp_predict": (C.c_int, [vp, f32p, C.c_int, f32p]),
+    "bsz_mlp_gradients": (C.c_int, [vp, f32p, f32p, C.c_int, f32p, C.POINTER(C.c_float)]),
+    "bsz_mlp_train_step": (C.c_int, [vp, f32p, f32p, C.c_int, C.c_float, C.POINTER(C.c_float)]),
+    "bsz_mlp_train": (C.c_int, [vp, f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64,
+                                C.POINTER(C.c_float)]),
 }
 
 _lib = None

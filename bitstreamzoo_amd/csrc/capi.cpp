@@ -6,6 +6,7 @@
 #include "executor.h"
 #include "fused.h"
 #include "frontend.h"
+#include "mlp.h"
 #include "run_json.h"
 
 #include <chrono>
@@ -381,8 +382,8 @@ int64_t bsz_plan_compile_only(const bsz_program* p, const bsz_case* shape, const
                               uint64_t cap) {
     std::string j;
     int rc = guard(nullptr, [&] {
-        auto fp = FusedPlan::record(nullptr, *p->prog, shape->c,
-                                    FusedOptions::from_json(options_json ? options_json : ""), true);
+        const FusedOptions o = FusedOptions::from_json(options_json ? options_json : "");
+        auto fp = FusedPlan::record(nullptr, *p->prog, shape->c, o, o.compile != 0);
         j = fp->info_json();
     });
     if (rc != BSZ_OK) return rc;
@@ -475,6 +476,75 @@ int bsz_graph_destroy(void* graph_exec) {
 int64_t bsz_plan_info(const bsz_plan* plan, char* buf, uint64_t cap) { return put_str(plan->plan->info_json(), buf, cap); }
 int64_t bsz_plan_source(const bsz_plan* plan, char* buf, uint64_t cap) {
     return put_str(plan->plan->module_source(), buf, cap);
+}
+
+}  // extern "C"
+
+// ── schedule cost model ──────────────────────────────────────────────────
+
+struct bsz_mlp {
+    bsz_ctx* ctx = nullptr;
+    std::unique_ptr<Mlp> m;
+};
+
+extern "C" {
+
+int bsz_mlp_create(bsz_ctx* ctx, int n_in, int h1, int h2, uint64_t seed, bsz_mlp** out) {
+    if (!ctx || !out) return fail(ctx, BSZ_E_INVALID, "null argument");
+    return guard(ctx, [&] {
+        auto* m = new bsz_mlp;
+        m->ctx = ctx;
+        try {
+            m->m = std::make_unique<Mlp>(ctx->s, n_in, h1, h2, seed);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int bsz_mlp_free(bsz_mlp* m) {
+    if (!m) return BSZ_OK;
+    int rc = guard(m->ctx, [&] { m->m.reset(); });
+    delete m;
+    return rc;
+}
+
+int64_t bsz_mlp_n_params(const bsz_mlp* m) { return m ? (int64_t)m->m->n_params() : 0; }
+
+int bsz_mlp_get_params(bsz_mlp* m, float* out) {
+    return guard(m->ctx, [&] { m->m->get_params(out); });
+}
+
+int bsz_mlp_set_params(bsz_mlp* m, const float* in) {
+    return guard(m->ctx, [&] { m->m->set_params(in); });
+}
+
+int bsz_mlp_predict(bsz_mlp* m, const float* x, int n, float* out) {
+    return guard(m->ctx, [&] { m->m->predict(x, n, out); });
+}
+
+int bsz_mlp_gradients(bsz_mlp* m, const float* x, const float* y, int n, float* grad, float* loss) {
+    return guard(m->ctx, [&] {
+        const float l = m->m->gradients(x, y, n, grad);
+        if (loss) *loss = l;
+    });
+}
+
+int bsz_mlp_train_step(bsz_mlp* m, const float* x, const float* y, int n, float lr, float* loss) {
+    return guard(m->ctx, [&] {
+        const float l = m->m->train_step(x, y, n, lr);
+        if (loss) *loss = l;
+    });
+}
+
+int bsz_mlp_train(bsz_mlp* m, const float* x, const float* y, int n, int epochs, int batch, float lr, uint64_t seed,
+                  float* loss) {
+    return guard(m->ctx, [&] {
+        const float l = m->m->train(x, y, n, epochs, batch, lr, seed);
+        if (loss) *loss = l;
+    });
 }
 
 }  // extern "C"

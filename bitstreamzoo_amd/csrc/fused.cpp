@@ -64,6 +64,9 @@ FusedOptions FusedOptions::from_json(const std::string& text) {
     get("max_ops", o.max_ops);
     get("max_live", o.max_live);
     get("occupancy", o.occupancy);
+    get("compile", o.compile);
+    get("compile_shard", o.compile_shard);
+    get("compile_shards", o.compile_shards);
     if (const Json* v = j.get("cache_dir")) o.cache_dir = v->s;
     return o;
 }
@@ -150,11 +153,13 @@ public:
     }
 
     // Compiles (or finds cached) every kernel unit of `p`; no device needed.
-    void compile_all(const PlanSpec& p, double* compile_ms) {
+    void compile_all(const PlanSpec& p, double* compile_ms, int shard = 0, int shards = 1) {
         const std::string arch = default_arch();
         std::vector<const KernelSpec*> todo;
         std::set<uint64_t> seen;
-        for (const KernelSpec& k : p.kernels) {
+        for (size_t ki = 0; ki < p.kernels.size(); ++ki) {
+            const KernelSpec& k = p.kernels[ki];
+            if (shards > 1 && (int)(ki % (size_t)shards) != shard) continue;
             const uint64_t h = unit_hash(k.unit, arch);
             dump(k.unit, h);
             if (seen.count(h) || have_code(h)) continue;
@@ -993,7 +998,9 @@ std::unique_ptr<FusedPlan> FusedPlan::record(hipStream_t s, const Program& prog,
         I.ints.push_back(x);
     }
     I.spec = std::make_shared<PlanSpec>(build_plan(g, to_plan_options(opt)));
-    if (compile && !I.spec->kernels.empty()) I.rec->jit().compile_all(*I.spec, &I.rec->compile_ms());
+    if (compile && !I.spec->kernels.empty())
+        I.rec->jit().compile_all(*I.spec, &I.rec->compile_ms(), std::max(0, opt.compile_shard),
+                                 std::max(1, opt.compile_shards));
     return fp;
 }
 

@@ -17,6 +17,9 @@ struct FusedOptions {
     int max_live = 96;          // live 64-bit values per lane per kernel
     int occupancy = 0;          // waves per SIMD the register budget allows (512/occupancy VGPRs); 0 = auto
     std::string cache_dir;      // code-object cache ("" = $BSZ_CACHE_DIR or ~/.cache/bsz_hip)
+    int compile = 1;            // compile-only: 0 partitions and generates without running hipRTC
+    int compile_shard = 0;      // compile-only: compile the kernels k with k % compile_shards == compile_shard
+    int compile_shards = 1;     //   (several processes fill the cache of one plan in parallel)
     bool verbose = false;
     static FusedOptions from_env();
     // Env defaults overridden by the keys of a flat JSON object

@@ -25,6 +25,45 @@ from . import api
 from .bsdata import BsCase
 
 
+def _shape_desc(cases: list[BsCase]) -> tuple:
+    c0 = cases[0]
+    return (dict(c0.params), sorted(c0.inputs), {k: sorted(a) for k, a in c0.input_arrays.items()})
+
+
+def _shape_from_desc(desc) -> api.Case:
+    params, inputs, arrays = desc
+    small = BsCase(bitlength=64, params=dict(params), inputs={k: np.zeros(1, np.uint64) for k in inputs},
+                   input_arrays={k: {i: np.zeros(1, np.uint64) for i in idx} for k, idx in arrays.items()})
+    return api.Case.from_bscase(small)
+
+
+def _compile_worker(job):
+    source, desc, options = job
+    api.compile_plan_only(api.Program(source), _shape_from_desc(desc), options)
+    return True
+
+
+def precompile(source: str, cases: list[BsCase], options_list: list[dict], processes: int = 16) -> float:
+    """Fill the code-object cache for every schedule of `options_list` with
+    `processes` worker processes (hipRTC compiles one unit at a time per
+    process); each plan's kernels are sharded over the workers.  Returns the
+    wall seconds spent.  Needs no GPU."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    t0 = time.perf_counter()
+    desc = _shape_desc(cases)
+    jobs = []
+    for opts in options_list:
+        for r in range(processes):
+            o = dict(opts)
+            o.update(compile_shard=r, compile_shards=processes)
+            jobs.append((source, desc, o))
+    with cf.ProcessPoolExecutor(max_workers=processes, mp_context=mp.get_context("spawn")) as ex:
+        for _ in ex.map(_compile_worker, jobs):
+            pass
+    return time.perf_counter() - t0
+
+
 def shape_case(cases: list[BsCase]) -> api.Case:
     """A 64-bit case with the params and input names of `cases[0]`: enough to
     record a straight-line program."""

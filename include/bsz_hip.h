@@ -137,7 +137,9 @@ int64_t bsz_result_json(const bsz_result* r, char* buf, uint64_t cap);
 int bsz_plan_create(bsz_ctx* ctx, const bsz_program* p, const bsz_case* shape, const char* options_json,
                     bsz_plan** out);
 /* Record + generate + hipRTC-compile only (no device needed); writes the plan
- * description JSON. */
+ * description JSON.  Option "compile":0 stops before hipRTC (partition and
+ * kernel shapes only); "compile_shard"/"compile_shards" compile a subset of the
+ * kernels, so several processes can fill the code-object cache in parallel. */
 int64_t bsz_plan_compile_only(const bsz_program* p, const bsz_case* shape, const char* options_json,
                               char* info, uint64_t cap);
 int bsz_plan_free(bsz_plan* plan);
@@ -175,6 +177,30 @@ int bsz_capture_begin(bsz_ctx* ctx);
 int bsz_capture_end(bsz_ctx* ctx, void** graph_exec);
 int bsz_graph_launch(bsz_ctx* ctx, void* graph_exec, int times);
 int bsz_graph_destroy(void* graph_exec);
+
+/* ── schedule cost model: a small MLP (n_in -> h1 -> h2 -> 1, ReLU) trained
+ *    and evaluated on the GPU with BF16 MFMA kernels (FP32 accumulation,
+ *    FP32 master weights, Adam).  It predicts the log run time of a
+ *    (program, schedule) pair from partition features, to rank schedules of
+ *    programs too large to autotune; it never touches program outputs.  The
+ *    reference has no schedule search (one kernel per op, kernels.cuh:7-36),
+ *    so these have no counterpart there. ─────────────────────────────────── */
+
+typedef struct bsz_mlp bsz_mlp;
+int bsz_mlp_create(bsz_ctx* ctx, int n_in, int h1, int h2, uint64_t seed, bsz_mlp** out);
+int bsz_mlp_free(bsz_mlp* m);
+/* Parameters as one float vector: W1[n_in][h1], b1[h1], W2[h1][h2], b2[h2], w3[h2], b3. */
+int64_t bsz_mlp_n_params(const bsz_mlp* m);
+int bsz_mlp_get_params(bsz_mlp* m, float* out);
+int bsz_mlp_set_params(bsz_mlp* m, const float* in);
+/* x: n rows of n_in floats (row-major); synchronous. */
+int bsz_mlp_predict(bsz_mlp* m, const float* x, int n, float* out);
+/* Mean squared error on a batch and its gradient (parameter layout), no update. */
+int bsz_mlp_gradients(bsz_mlp* m, const float* x, const float* y, int n, float* grad, float* loss);
+int bsz_mlp_train_step(bsz_mlp* m, const float* x, const float* y, int n, float lr, float* loss);
+/* `epochs` shuffled passes in minibatches of `batch` rows; *loss = last epoch's mean batch loss. */
+int bsz_mlp_train(bsz_mlp* m, const float* x, const float* y, int n, int epochs, int batch, float lr, uint64_t seed,
+                  float* loss);
 
 #ifdef __cplusplus
 }
