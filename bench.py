@@ -233,6 +233,28 @@ def bench_shift_and(ctx, args):
                "plan": info, "instances": n, "checked_instances": len(check_idx), "options": opts, "tune": tune}
 
 
+def bench_learned(ctx, args):
+    """The learned cost model against the measured autotuner on held-out
+    generated DAG programs: train the GPU MLP on the committed autotuner
+    measurements, then per program compare the model's pick with the full
+    autotune over the same candidates (gap, wall-time ratio, prediction
+    error, gain over the untuned default schedule)."""
+    from bitstreamzoo_amd import costmodel, schedsearch
+    data = np.load(os.path.join(ROOT, "bitstreamzoo_amd", "data", "schedule_runtimes.npz"))
+    model = costmodel.CostModel(ctx, hidden=64, seed=0)
+    loss = model.fit(data["X"], data["ms"], epochs=300, batch=64, lr=2e-3)
+    log(f"learned: trained on {len(data['ms'])} autotuner measurements in {model.train_seconds:.2f} s (loss {loss:.4f})")
+    cands = schedsearch.sample_space(40, 12345)
+    held = schedsearch.random_specs(args.learned_programs, 7, nodes=(2000, 5000), log2_bytes=(23, 25))
+    rows = schedsearch.evaluate(ctx, model, held, cands, processes=min(16, os.cpu_count() or 1), log=log)
+    return {"train_rows": int(len(data["ms"])), "train_s": round(model.train_seconds, 3), "candidates": len(cands),
+            "programs": [f"{r.nodes} nodes x {r.n_bytes} B" for r in rows],
+            "gap_pct": [round(100 * r.gap, 2) for r in rows],
+            "time_ratio_pct": [round(100 * r.model_s / r.tune_s, 2) for r in rows],
+            "mape_pct": [round(100 * r.mape, 1) for r in rows],
+            "pick_vs_default_speedup": [round(r.default_ms / r.pick_ms, 3) for r in rows]}
+
+
 def copy_bandwidth(ctx) -> float:
     """Achieved GB/s of a plain device copy (read + write) of 2 GiB."""
     bl = 1 << 34  # 2 GiB per buffer
@@ -281,6 +303,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner; use fixed default schedules")
     ap.add_argument("--schedule", default="", help="JSON options for the measured workload's plan (skips tuning)")
+    ap.add_argument("--learned-programs", type=int, default=2,
+                    help="held-out programs for the learned-vs-autotuned comparison (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the executor runs on a single MI355X and this build has no "
@@ -306,6 +330,14 @@ def main():
                 raise
             except Exception as e:  # report, do not hide
                 extra[name] = {"error": str(e)}
+    learned = None
+    if not args.no_extra and args.workload == "adder32" and args.learned_programs > 0:
+        try:
+            learned = bench_learned(ctx, args)
+        except SystemExit:
+            raise
+        except Exception as e:  # report, do not hide
+            learned = {"error": str(e)}
     copy_gbps = copy_bandwidth(ctx)
 
     ms = r["ms_per_step"]
@@ -355,6 +387,8 @@ def main():
                     "speedup_fused_over_perop": round(rr["perop_ms_per_step"] / m, 2)}
     if ex:
         line["other_configs"] = ex
+    if learned is not None:
+        line["learned_vs_autotuned"] = learned
     if not args.no_cpu:
         log("cpu baseline: oracle AVX-512+OpenMP on a 2^24-bit adder32 sample")
         line["cpu_baseline"] = cpu_baseline(1 << 24)

@@ -70,6 +70,7 @@ _SIGS = {
     "bsz_plan_create": (C.c_int, [vp, vp, vp, C.c_char_p, C.POINTER(vp)]),
     "bsz_plan_compile_only": (C.c_int64, [vp, vp, C.c_char_p, C.c_char_p, C.c_uint64]),
     "bsz_plan_free": (C.c_int, [vp]),
+    "bsz_plan_describe": (C.c_int64, [vp, vp, C.c_char_p, C.c_char_p, C.c_uint64]),
     "bsz_plan_set_instances": (C.c_int, [vp, C.c_int, i64p]),
     "bsz_plan_bind_input": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int64, u64p]),
     "bsz_plan_output": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int64, C.POINTER(u64p)]),

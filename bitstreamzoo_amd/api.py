@@ -268,6 +268,18 @@ def compile_plan_only(program: "Program", shape: "Case", options: dict | None = 
     return json.loads(buf.value.decode())
 
 
+def describe_plans(program: "Program", shape: "Case", options_list: list[dict]) -> list[dict]:
+    """Kernel shapes of the program under each schedule (no code, no device)."""
+    js = json.dumps(options_list).encode()
+    L = lib()
+    n = L.bsz_plan_describe(program.h, shape.h, js, None, 0)
+    if n < 0:
+        check(int(n))
+    buf = C.create_string_buffer(n + 1)
+    L.bsz_plan_describe(program.h, shape.h, js, buf, n + 1)
+    return json.loads(buf.value.decode())
+
+
 class Plan:
     """A fused plan over a batch of instances (bsz_plan_*)."""
 

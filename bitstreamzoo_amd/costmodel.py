@@ -5,7 +5,8 @@ program under a candidate schedule from features of the fused plan that
 schedule produces: the partition (kernels, nodes, stateful ops, streams read
 and written), the tile configuration (rows per chunk, rows in flight,
 occupancy, workgroup width) and the problem size.  Features come from
-planning alone (`compile_plan_only(..., {"compile": 0})`, milliseconds), so
+planning alone (`bsz_plan_describe`: the program is recorded once and
+partitioned per schedule, no code generated; milliseconds per candidate), so
 ranking a few dozen candidates costs far less than compiling and timing them.
 Training and inference run on the GPU in libbsz_hip.so (BF16 MFMA products,
 FP32 accumulation, Adam).  The measured autotuner (tuning.py) produces the
@@ -68,10 +69,15 @@ def plan_features(info: dict, options: dict, words_per_instance: float, n_instan
 
 def features_for(program: api.Program, shape: api.Case, options: dict, words_per_instance: float,
                  n_instances: int) -> np.ndarray:
-    o = dict(options)
-    o["compile"] = 0
-    info = api.compile_plan_only(program, shape, o)
-    return plan_features(info, options, words_per_instance, n_instances)
+    return features_batch(program, shape, [options], words_per_instance, n_instances)[0]
+
+
+def features_batch(program: api.Program, shape: api.Case, options_list: list[dict], words_per_instance: float,
+                   n_instances: int) -> np.ndarray:
+    """Features of every schedule of `options_list`: the program is recorded
+    once and partitioned per schedule (bsz_plan_describe), no code generated."""
+    infos = api.describe_plans(program, shape, options_list)
+    return np.stack([plan_features(info, o, words_per_instance, n_instances) for info, o in zip(infos, options_list)])
 
 
 class GpuMlp:

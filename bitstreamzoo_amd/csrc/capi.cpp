@@ -390,6 +390,18 @@ int64_t bsz_plan_compile_only(const bsz_program* p, const bsz_case* shape, const
     return put_str(j, info, cap);
 }
 
+int64_t bsz_plan_describe(const bsz_program* p, const bsz_case* shape, const char* options_array_json, char* out,
+                          uint64_t cap) {
+    std::string j;
+    int rc = guard(nullptr, [&] {
+        FusedOptions o = FusedOptions::from_env();
+        auto fp = FusedPlan::record(nullptr, *p->prog, shape->c, o, false);
+        j = fp->describe_json(options_array_json ? options_array_json : "[]");
+    });
+    if (rc != BSZ_OK) return rc;
+    return put_str(j, out, cap);
+}
+
 int bsz_plan_free(bsz_plan* plan) {
     if (!plan) return BSZ_OK;
     int rc = guard(plan->ctx, [&] { plan->plan.reset(); });

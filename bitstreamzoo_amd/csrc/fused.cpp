@@ -51,9 +51,12 @@ FusedOptions FusedOptions::from_env() {
 }
 
 FusedOptions FusedOptions::from_json(const std::string& text) {
+    if (text.empty()) return from_env();
+    return from_json_obj(json_parse(text));
+}
+
+FusedOptions FusedOptions::from_json_obj(const Json& j) {
     FusedOptions o = from_env();
-    if (text.empty()) return o;
-    Json j = json_parse(text);
     auto get = [&](const char* k, int& dst) {
         if (const Json* v = j.get(k)) dst = (int)v->as_int(dst);
     };
@@ -1197,6 +1200,27 @@ int64_t FusedPlan::read_int(int inst, const std::string& name, bool refresh) {
 }
 
 uint32_t FusedPlan::mismatches(int kernel) { return impl_->rt->mismatches(impl_->s, kernel); }
+
+std::string FusedPlan::describe_json(const std::string& options_array) const {
+    const Json arr = json_parse(options_array);
+    if (arr.k != Json::Arr) throw RunError("describe: expected a JSON array of option objects");
+    std::ostringstream o;
+    o << "[";
+    for (size_t i = 0; i < arr.a.size(); ++i) {
+        const PlanSpec ps = build_plan(impl_->spec->g, to_plan_options(FusedOptions::from_json_obj(arr.a[i])), false);
+        o << (i ? "," : "") << "{\"kernels\":[";
+        for (size_t k = 0; k < ps.kernels.size(); ++k) {
+            const KernelSpec& ks = ps.kernels[k];
+            o << (k ? "," : "") << "{\"nodes\":" << ks.nodes.size() << ",\"inputs\":" << ks.inputs.size()
+              << ",\"outputs\":" << ks.outputs.size() << ",\"pops\":" << ks.pops.size()
+              << ",\"state_words\":" << ks.n_state << ",\"rows_per_step\":" << ks.rows_per_step
+              << ",\"occupancy\":" << ks.occupancy << "}";
+        }
+        o << "],\"slots\":" << ps.n_slots << ",\"chunk_rows\":" << ps.opt.chunk_rows << "}";
+    }
+    o << "]";
+    return o.str();
+}
 
 std::string FusedPlan::info_json() const {
     const Impl& I = *impl_;

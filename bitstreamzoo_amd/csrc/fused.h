@@ -8,6 +8,8 @@
 
 namespace bsz {
 
+struct Json;
+
 struct FusedOptions {
     int chunk_rows = 16;        // 64-word rows per wave-chunk
     int warmup_rows = 1;        // speculative warm-up rows before a chunk
@@ -25,6 +27,7 @@ struct FusedOptions {
     // Env defaults overridden by the keys of a flat JSON object
     // ({"chunk_rows":16,"warmup_rows":1,...}); "" leaves the defaults.
     static FusedOptions from_json(const std::string& json);
+    static FusedOptions from_json_obj(const Json& j);
 };
 
 std::unique_ptr<Engine> make_fused_engine(hipStream_t s, const FusedOptions& opt);
@@ -56,6 +59,9 @@ public:
     int64_t read_int(int inst, const std::string& name, bool refresh);
     uint32_t mismatches(int kernel);
     std::string info_json() const;
+    // Kernel shapes of the recorded program under each schedule of a JSON
+    // array of option objects (partitioning only, no code generation).
+    std::string describe_json(const std::string& options_array) const;
     const std::string& module_source() const;
     int64_t op_count() const;
     int n_kernels() const;

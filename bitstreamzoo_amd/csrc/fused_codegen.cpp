@@ -137,7 +137,7 @@ std::string hex64(uint64_t v) {
 }
 
 // Generates one kernel; fills ks.source and ks.n_state.
-void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
+void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool emit) {
     std::set<int32_t> in_kernel(ks.nodes.begin(), ks.nodes.end());
     std::map<int32_t, int> input_index;     // graph node -> kernel input slot
     for (size_t j = 0; j < ks.input_node.size(); ++j) input_index[ks.input_node[j]] = (int)j;
@@ -224,6 +224,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks) {
     const int WPB = opt.waves_per_block;
     const int LB = has_state ? std::max(64 * WPB, kFixThreads) : 64 * WPB;
     ks.launch_bounds = LB;
+    if (!emit) return;  // shape only (register budget, rows per step), no source
     const std::string NS = std::to_string(n_state) + "u";
     s << "extern \"C\" __global__ __launch_bounds__(" << LB << ") __attribute__((amdgpu_waves_per_eu(" << occ
       << "))) void bsz_kernel(KArgs A) {\n";
@@ -538,7 +539,7 @@ std::string PlanOptions::key() const {
     return s.str();
 }
 
-PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt) {
+PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
     PlanSpec P;
     P.g = g;
     P.opt = opt;
@@ -698,7 +699,8 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt) {
     std::ostringstream mod;
     mod << kKernelPrelude;
     for (KernelSpec& ks : P.kernels) {
-        gen_kernel(g, opt, ks);
+        gen_kernel(g, opt, ks, emit);
+        if (!emit) continue;
         ks.unit = std::string(kKernelPrelude) + ks.source;
         mod << "// ---- " << ks.name << " ----\n" << ks.source;
     }

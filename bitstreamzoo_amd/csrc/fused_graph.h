@@ -102,7 +102,9 @@ struct PlanSpec {
 };
 
 // Dead-code elimination, partitioning, slot assignment and code generation.
-PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt);
+// emit=false stops after the kernel shapes (no HIP source): enough to
+// describe a candidate schedule.
+PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit = true);
 
 // Argument block every generated kernel takes by value (layout shared with the
 // generated source, see kKernelPrelude in fused_codegen.cpp).

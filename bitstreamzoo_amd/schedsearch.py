@@ -1,0 +1,229 @@
+"""Schedule search for large generated programs: the measured autotuner as
+data source and judge, the learned cost model as the fast path.
+
+  collect()   seeded generator DAGs of varied size over Zipf texts of varied
+              length; for each, a sample of candidate schedules (partition
+              knobs max_live / max_ops and tile knobs chunk_rows /
+              rows_per_step / occupancy) is compiled (in parallel worker
+              processes), checked bit-exact against the first candidate and
+              timed with hipEvents; rows = (plan features, measured ms).
+  evaluate()  on held-out programs: the full autotune over the candidate
+              space (its wall time and best schedule) against the model's
+              pick (plan-only features for every candidate, GPU inference,
+              then compile + time of the top pick only): the pick's gap to
+              the measured best, the time ratio, and the model's
+              predicted-vs-measured error on every candidate.
+
+Nothing here changes program outputs: every schedule is checked against the
+others and the model only chooses which one to build.
+"""
+
+from __future__ import annotations
+
+import itertools
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import api, costmodel, dag_gen, tuning
+
+SPACE_GRID = {"max_live": [32, 64, 96, 160], "max_ops": [250, 1000, 4000], "chunk_rows": [4, 16, 64],
+              "rows_per_step": [1, 4], "occupancy": [0, 1]}
+
+
+def full_space() -> list[dict]:
+    keys = sorted(SPACE_GRID)
+    return [dict(zip(keys, v)) for v in itertools.product(*(SPACE_GRID[k] for k in keys))]
+
+
+def sample_space(n: int, seed: int) -> list[dict]:
+    sp = full_space()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    idx = rng.choice(len(sp), size=min(n, len(sp)), replace=False)
+    return [sp[i] for i in sorted(idx)]
+
+
+@dataclass
+class ProgramSpec:
+    nodes: int
+    n_bytes: int
+    seed: int
+
+    def build(self):
+        return dag_gen.generate(self.nodes, seed=self.seed), dag_gen.text_case(self.n_bytes, seed=self.seed + 7)
+
+
+def random_specs(n: int, seed: int, nodes=(300, 3000), log2_bytes=(20, 24)) -> list[ProgramSpec]:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n):
+        nn = int(math.exp(rng.uniform(math.log(nodes[0]), math.log(nodes[1]))))
+        nb = int(2 ** rng.uniform(*log2_bytes))
+        out.append(ProgramSpec(nn, nb, seed * 1000 + i))
+    return out
+
+
+@dataclass
+class Measured:
+    options: list = field(default_factory=list)
+    ms: list = field(default_factory=list)
+    features: list = field(default_factory=list)
+    compile_s: float = 0.0
+    measure_s: float = 0.0
+
+
+def measure(ctx: api.Context, source: str, case, candidates: list[dict], steps: int = 5, processes: int = 16,
+            log=None) -> Measured:
+    """Compile every candidate (parallel processes), then check and time each."""
+    out = Measured()
+    t0 = time.perf_counter()
+    tuning.precompile(source, [case], candidates, processes=processes)
+    out.compile_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    prog = api.Program(source)
+    shape = tuning.shape_case([case])
+    res = tuning.Resident(ctx, [case])
+    names = [n for n in source.split("\n") if n.startswith("output stream ")]
+    outs = [n.split()[-1] for n in names][:8]
+    try:
+        tr = tuning.autotune(ctx, prog, [res], candidates, [], outs, steps=steps, warmup=1, log=log)
+    finally:
+        res.free()
+    words = max(1, (case.bitlength + 63) // 64)
+    F = costmodel.features_batch(prog, shape, [t.options for t in tr.trials], words, 1)
+    for t, f in zip(tr.trials, F):
+        out.options.append(t.options)
+        out.ms.append(t.ms)
+        out.features.append(f)
+    out.measure_s = time.perf_counter() - t1
+    return out
+
+
+def collect(ctx: api.Context, specs: list[ProgramSpec], per_program: int = 16, seed: int = 0, processes: int = 16,
+            log=None):
+    """Training rows (features, ms, program index) from the measured tuner."""
+    X, y, gid = [], [], []
+    for i, sp in enumerate(specs):
+        p, case = sp.build()
+        cands = sample_space(per_program, seed + i)
+        m = measure(ctx, p.source, case, cands, processes=processes)
+        X += m.features
+        y += m.ms
+        gid += [i] * len(m.ms)
+        if log:
+            log(f"collect {i + 1}/{len(specs)}: {sp.nodes} nodes, {sp.n_bytes} bytes, best {min(m.ms):.3f} ms, "
+                f"worst {max(m.ms):.3f} ms (compile {m.compile_s:.1f} s, measure {m.measure_s:.1f} s)")
+    return np.array(X, dtype=np.float32), np.array(y, dtype=np.float64), np.array(gid)
+
+
+@dataclass
+class EvalRow:
+    nodes: int
+    n_bytes: int
+    best_ms: float
+    pick_ms: float
+    default_ms: float
+    tune_s: float
+    model_s: float
+    mape: float
+
+    @property
+    def gap(self) -> float:
+        return self.pick_ms / self.best_ms - 1.0
+
+
+def evaluate(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSpec], candidates: list[dict],
+             processes: int = 16, default: dict | None = None, log=None) -> list[EvalRow]:
+    """Learned pick against the full measured autotune, per held-out program."""
+    default = default if default is not None else {}
+    rows = []
+    for sp in specs:
+        p, case = sp.build()
+        # the model first (so its compile finds nothing cached): plan-only
+        # features of every candidate, GPU inference, then compile and time
+        # only the top pick
+        t1 = time.perf_counter()
+        prog = api.Program(p.source)
+        shape = tuning.shape_case([case])
+        F = costmodel.features_batch(prog, shape, candidates, max(1, (case.bitlength + 63) // 64), 1)
+        pred = model.predict_ms(F)
+        k = int(np.argmin(pred))
+        feat_s = time.perf_counter() - t1
+        pick = measure(ctx, p.source, case, [candidates[k]], processes=processes)
+        model_s = time.perf_counter() - t1
+        # the autotuner: compile + check + time every candidate
+        t0 = time.perf_counter()
+        m = measure(ctx, p.source, case, candidates, processes=processes)
+        tune_s = time.perf_counter() - t0
+        best = min(m.ms)
+        # the schedule without any search, for the ablation
+        d = measure(ctx, p.source, case, [default], processes=processes)
+        mape = float(np.mean(np.abs(pred - np.array(m.ms)) / np.array(m.ms)))
+        row = EvalRow(sp.nodes, sp.n_bytes, best, pick.ms[0], d.ms[0], tune_s, model_s, mape)
+        rows.append(row)
+        if log:
+            log(f"eval {sp.nodes} nodes / {sp.n_bytes} B: best {best:.3f} ms, pick {row.pick_ms:.3f} ms "
+                f"(gap {100 * row.gap:.1f} %), default {row.default_ms:.3f} ms, tune {tune_s:.1f} s, "
+                f"model {model_s:.1f} s (features + inference {feat_s:.2f} s, pick compile {pick.compile_s:.2f} s, "
+                f"pick measure {pick.measure_s:.2f} s), MAPE {100 * mape:.1f} %")
+    return rows
+
+
+def main(argv=None) -> int:
+    """python -m bitstreamzoo_amd.schedsearch --out FILE.npz [--programs N] [--per-program K]
+    [--eval N] [--candidates K] [--processes P]: collect training rows with the
+    measured autotuner, train the cost model on the GPU, evaluate it on
+    held-out programs and print one JSON summary line."""
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default="")
+    ap.add_argument("--data", default="", help="reuse collected rows instead of collecting")
+    ap.add_argument("--programs", type=int, default=40)
+    ap.add_argument("--per-program", type=int, default=16)
+    ap.add_argument("--eval", type=int, default=4)
+    ap.add_argument("--candidates", type=int, default=40)
+    ap.add_argument("--processes", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args(argv)
+
+    def log(m):
+        print(f"[schedsearch] {m}", file=sys.stderr, flush=True)
+
+    ctx = api.Context(0)
+    if a.data:
+        d = np.load(a.data)
+        X, y, gid = d["X"], d["ms"], d["gid"]
+    else:
+        specs = random_specs(a.programs, a.seed)
+        X, y, gid = collect(ctx, specs, a.per_program, seed=a.seed, processes=a.processes, log=log)
+        if a.out:
+            np.savez(a.out, X=X, ms=y, gid=gid, features=np.array(costmodel.FEATURES))
+    model = costmodel.CostModel(ctx, hidden=64, seed=0)
+    loss = model.fit(X, y, epochs=300, batch=64, lr=2e-3)
+    log(f"trained on {len(y)} rows in {model.train_seconds:.2f} s, loss {loss:.4f}")
+    cands = sample_space(a.candidates, 12345)
+    held = random_specs(a.eval, a.seed + 100, nodes=(2000, 5000), log2_bytes=(23, 25))
+    rows = evaluate(ctx, model, held, cands, processes=a.processes, log=log)
+    summary = {
+        "train_rows": int(len(y)), "train_programs": int(len(set(gid.tolist()))), "train_s": round(model.train_seconds, 2),
+        "candidates": len(cands),
+        "gap_pct": [round(100 * r.gap, 2) for r in rows],
+        "time_ratio_pct": [round(100 * r.model_s / r.tune_s, 2) for r in rows],
+        "mape_pct": [round(100 * r.mape, 1) for r in rows],
+        "pick_vs_default_speedup": [round(r.default_ms / r.pick_ms, 3) for r in rows],
+        "programs": [{"nodes": r.nodes, "bytes": r.n_bytes, "best_ms": round(r.best_ms, 4), "pick_ms": round(r.pick_ms, 4),
+                      "default_ms": round(r.default_ms, 4), "tune_s": round(r.tune_s, 1), "model_s": round(r.model_s, 2)}
+                     for r in rows],
+    }
+    print(json.dumps(summary), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -48,8 +48,6 @@ def precompile(source: str, cases: list[BsCase], options_list: list[dict], proce
     `processes` worker processes (hipRTC compiles one unit at a time per
     process); each plan's kernels are sharded over the workers.  Returns the
     wall seconds spent.  Needs no GPU."""
-    import concurrent.futures as cf
-    import multiprocessing as mp
     t0 = time.perf_counter()
     desc = _shape_desc(cases)
     jobs = []
@@ -58,10 +56,25 @@ def precompile(source: str, cases: list[BsCase], options_list: list[dict], proce
             o = dict(opts)
             o.update(compile_shard=r, compile_shards=processes)
             jobs.append((source, desc, o))
-    with cf.ProcessPoolExecutor(max_workers=processes, mp_context=mp.get_context("spawn")) as ex:
-        for _ in ex.map(_compile_worker, jobs):
-            pass
+    for _ in _pool(processes).map(_compile_worker, jobs):
+        pass
     return time.perf_counter() - t0
+
+
+_POOL = None
+
+
+def _pool(processes: int):
+    """Worker processes for compilation, started once (spawned: they never
+    touch the GPU) and reused by every precompile() of this process."""
+    global _POOL
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    if _POOL is None or _POOL[0] != processes:
+        if _POOL is not None:
+            _POOL[1].shutdown()
+        _POOL = (processes, cf.ProcessPoolExecutor(max_workers=processes, mp_context=mp.get_context("spawn")))
+    return _POOL[1]
 
 
 def shape_case(cases: list[BsCase]) -> api.Case:

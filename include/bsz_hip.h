@@ -142,6 +142,12 @@ int bsz_plan_create(bsz_ctx* ctx, const bsz_program* p, const bsz_case* shape, c
  * kernels, so several processes can fill the code-object cache in parallel. */
 int64_t bsz_plan_compile_only(const bsz_program* p, const bsz_case* shape, const char* options_json,
                               char* info, uint64_t cap);
+/* Kernel shapes (per kernel: nodes, inputs, outputs, popcounts, state words,
+ * rows per step, occupancy) of the program under each schedule of a JSON
+ * array of option objects; partitioning only (no code, no device).  Writes a
+ * JSON array, one element per schedule.  Features for the cost model. */
+int64_t bsz_plan_describe(const bsz_program* p, const bsz_case* shape, const char* options_array_json, char* out,
+                          uint64_t cap);
 int bsz_plan_free(bsz_plan* plan);
 /* One instance per bitlength; allocates the plan's output and scratch buffers. */
 int bsz_plan_set_instances(bsz_plan* plan, int n, const int64_t* bitlengths);

@@ -83,3 +83,17 @@ def test_reference_straight_line_programs_compile():
         except BszError as e:
             assert e.code == -3, (cid, str(e)[:2000])
     assert n_ok >= 5
+
+
+def test_describe_matches_compile_only_shapes():
+    from bitstreamzoo_amd import dag_gen
+    p = dag_gen.generate(700, seed=4)
+    case = dag_gen.text_case(5000, seed=4)
+    prog, shape = api.Program(p.source), api.Case.from_bscase(case)
+    opts = [{"max_live": 32, "chunk_rows": 4}, {"max_ops": 100, "occupancy": 1, "rows_per_step": 4}, {}]
+    desc = api.describe_plans(prog, shape, opts)
+    assert len(desc) == len(opts)
+    for d, o in zip(desc, opts):
+        info = api.compile_plan_only(prog, shape, dict(o, compile=0))
+        keys = ("nodes", "inputs", "outputs", "pops", "state_words", "rows_per_step", "occupancy")
+        assert [[k[x] for x in keys] for k in d["kernels"]] == [[k[x] for x in keys] for k in info["kernels"]]
