@@ -240,14 +240,15 @@ def bench_learned(ctx, args):
     autotune over the same candidates (gap, wall-time ratio, prediction
     error, gain over the untuned default schedule)."""
     from bitstreamzoo_amd import costmodel, schedsearch
-    data = np.load(os.path.join(ROOT, "bitstreamzoo_amd", "data", "schedule_runtimes.npz"))
-    model = costmodel.CostModel(ctx, hidden=64, seed=0)
-    loss = model.fit(data["X"], data["ms"], epochs=300, batch=64, lr=2e-3)
-    log(f"learned: trained on {len(data['ms'])} autotuner measurements in {model.train_seconds:.2f} s (loss {loss:.4f})")
-    cands = schedsearch.sample_space(40, 12345)
+    X, ms = schedsearch.load_training_data()
+    model = costmodel.CostModel(ctx, seed=0)
+    loss = model.fit(X, ms)
+    log(f"learned: {model.members} networks trained on {len(ms)} autotuner measurements in "
+        f"{model.train_seconds:.2f} s (loss {loss:.4f})")
+    cands = schedsearch.sample_space(64, 12345)
     held = schedsearch.random_specs(args.learned_programs, 7, nodes=(2000, 5000), log2_bytes=(23, 25))
     rows = schedsearch.evaluate(ctx, model, held, cands, processes=min(16, os.cpu_count() or 1), log=log)
-    return {"train_rows": int(len(data["ms"])), "train_s": round(model.train_seconds, 3), "candidates": len(cands),
+    return {"train_rows": int(len(ms)), "train_s": round(model.train_seconds, 3), "candidates": len(cands),
             "programs": [f"{r.nodes} nodes x {r.n_bytes} B" for r in rows],
             "gap_pct": [round(100 * r.gap, 2) for r in rows],
             "time_ratio_pct": [round(100 * r.model_s / r.tune_s, 2) for r in rows],

@@ -150,36 +150,47 @@ class GpuMlp:
 
 @dataclass
 class CostModel:
-    """Standardized features -> GpuMlp -> standardized log(ms)."""
+    """Standardized features -> an ensemble of GpuMlp -> standardized log(ms).
+
+    Members differ only in their initialisation and minibatch order; the
+    prediction is their mean, which generalizes better to programs larger
+    than the training ones than one network trained longer."""
     ctx: api.Context
-    hidden: int = 64
+    hidden: int = 32
+    members: int = 5
     seed: int = 0
     mu: np.ndarray = field(default=None)
     sd: np.ndarray = field(default=None)
     y_mu: float = 0.0
     y_sd: float = 1.0
-    net: GpuMlp = field(default=None)
+    nets: list = field(default_factory=list)
     train_seconds: float = 0.0
 
-    def fit(self, X: np.ndarray, ms: np.ndarray, epochs: int = 300, batch: int = 128, lr: float = 2e-3) -> float:
+    def fit(self, X: np.ndarray, ms: np.ndarray, epochs: int = 80, batch: int = 64, lr: float = 2e-3) -> float:
         t0 = time.perf_counter()
         X = np.asarray(X, dtype=np.float64)
         y = np.log(np.asarray(ms, dtype=np.float64))
         self.mu, self.sd = X.mean(axis=0), X.std(axis=0) + 1e-6
         self.y_mu, self.y_sd = float(y.mean()), float(y.std() + 1e-6)
-        self.net = GpuMlp(self.ctx, X.shape[1], self.hidden, self.hidden, self.seed)
-        loss = self.net.train((X - self.mu) / self.sd, (y - self.y_mu) / self.y_sd, epochs, batch, lr, self.seed)
+        Z, yz = (X - self.mu) / self.sd, (y - self.y_mu) / self.y_sd
+        for n in self.nets:
+            n.close()
+        self.nets, losses = [], []
+        for k in range(self.members):
+            net = GpuMlp(self.ctx, X.shape[1], self.hidden, self.hidden, self.seed * 101 + k)
+            losses.append(net.train(Z, yz, epochs, batch, lr, self.seed * 101 + k))
+            self.nets.append(net)
         self.train_seconds = time.perf_counter() - t0
-        return loss
+        return float(np.mean(losses))
 
     def predict_ms(self, X: np.ndarray) -> np.ndarray:
-        X = np.asarray(X, dtype=np.float64)
-        z = self.net.predict((X - self.mu) / self.sd)
-        return np.exp(z.astype(np.float64) * self.y_sd + self.y_mu)
+        Z = (np.asarray(X, dtype=np.float64) - self.mu) / self.sd
+        z = np.mean([n.predict(Z).astype(np.float64) for n in self.nets], axis=0)
+        return np.exp(z * self.y_sd + self.y_mu)
 
     def state(self) -> dict:
         return {"mu": self.mu.tolist(), "sd": self.sd.tolist(), "y_mu": self.y_mu, "y_sd": self.y_sd,
-                "hidden": self.hidden, "params": self.net.get_params().tolist()}
+                "hidden": self.hidden, "params": [n.get_params().tolist() for n in self.nets]}
 
     def save(self, path: str):
         with open(path, "w") as f:
@@ -189,9 +200,11 @@ class CostModel:
     def load(cls, ctx: api.Context, path: str) -> "CostModel":
         with open(path) as f:
             s = json.load(f)
-        m = cls(ctx, hidden=s["hidden"])
+        m = cls(ctx, hidden=s["hidden"], members=len(s["params"]))
         m.mu, m.sd = np.array(s["mu"]), np.array(s["sd"])
         m.y_mu, m.y_sd = s["y_mu"], s["y_sd"]
-        m.net = GpuMlp(ctx, len(m.mu), m.hidden, m.hidden)
-        m.net.set_params(np.array(s["params"], dtype=np.float32))
+        for p in s["params"]:
+            net = GpuMlp(ctx, len(m.mu), m.hidden, m.hidden)
+            net.set_params(np.array(p, dtype=np.float32))
+            m.nets.append(net)
         return m

@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import itertools
 import math
+import os
 import time
 from dataclasses import dataclass, field
 
@@ -171,6 +172,22 @@ def evaluate(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSp
     return rows
 
 
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def load_training_data():
+    """Every committed autotuner measurement file (data/schedule_runtimes*.npz)."""
+    import glob
+    X, ms = [], []
+    for f in sorted(glob.glob(os.path.join(DATA_DIR, "schedule_runtimes*.npz"))):
+        d = np.load(f)
+        X.append(d["X"])
+        ms.append(d["ms"])
+    if not X:
+        raise FileNotFoundError(f"no schedule_runtimes*.npz under {DATA_DIR}")
+    return np.concatenate(X), np.concatenate(ms)
+
+
 def main(argv=None) -> int:
     """python -m bitstreamzoo_amd.schedsearch --out FILE.npz [--programs N] [--per-program K]
     [--eval N] [--candidates K] [--processes P]: collect training rows with the
@@ -189,22 +206,34 @@ def main(argv=None) -> int:
     ap.add_argument("--candidates", type=int, default=40)
     ap.add_argument("--processes", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--min-nodes", type=int, default=300)
+    ap.add_argument("--max-nodes", type=int, default=3000)
+    ap.add_argument("--min-log2-bytes", type=float, default=20)
+    ap.add_argument("--max-log2-bytes", type=float, default=24)
+    ap.add_argument("--collect-only", action="store_true")
     a = ap.parse_args(argv)
 
     def log(m):
         print(f"[schedsearch] {m}", file=sys.stderr, flush=True)
 
     ctx = api.Context(0)
-    if a.data:
+    if a.data == "committed":
+        X, y = load_training_data()
+        gid = np.zeros(len(y), dtype=np.int64)
+    elif a.data:
         d = np.load(a.data)
         X, y, gid = d["X"], d["ms"], d["gid"]
     else:
-        specs = random_specs(a.programs, a.seed)
+        specs = random_specs(a.programs, a.seed, nodes=(a.min_nodes, a.max_nodes),
+                             log2_bytes=(a.min_log2_bytes, a.max_log2_bytes))
         X, y, gid = collect(ctx, specs, a.per_program, seed=a.seed, processes=a.processes, log=log)
         if a.out:
             np.savez(a.out, X=X, ms=y, gid=gid, features=np.array(costmodel.FEATURES))
-    model = costmodel.CostModel(ctx, hidden=64, seed=0)
-    loss = model.fit(X, y, epochs=300, batch=64, lr=2e-3)
+        if a.collect_only:
+            ctx.close()
+            return 0
+    model = costmodel.CostModel(ctx, seed=0)
+    loss = model.fit(X, y)
     log(f"trained on {len(y)} rows in {model.train_seconds:.2f} s, loss {loss:.4f}")
     cands = sample_space(a.candidates, 12345)
     held = random_specs(a.eval, a.seed + 100, nodes=(2000, 5000), log2_bytes=(23, 25))

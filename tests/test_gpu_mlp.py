@@ -81,7 +81,7 @@ def test_cost_model_ranks_by_a_known_law():
     rng = np.random.default_rng(5)
     X = rng.uniform(0, 4, size=(1500, 6))
     ms = np.exp(0.5 * X[:, 0] - 0.3 * X[:, 1] + 0.2 * X[:, 2] * X[:, 3] + 0.05 * rng.normal(size=1500))
-    m = CostModel(ctx(), hidden=32, seed=0)
+    m = CostModel(ctx(), hidden=32, members=3, seed=0)
     m.fit(X[:1200], ms[:1200], epochs=150, batch=64, lr=3e-3)
     pred = m.predict_ms(X[1200:])
     err = np.abs(np.log(pred) - np.log(ms[1200:]))
