@@ -114,13 +114,6 @@ __device__ __forceinline__ u32 aload32(u32* p) { return __hip_atomic_load(p, __A
 __device__ __forceinline__ u64 aload64(u64* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT); }
 __device__ __forceinline__ void astore32(u32* p, u32 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT); }
 __device__ __forceinline__ void astore64(u64* p, u64 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT); }
-// Wait until this wave's outstanding memory operations completed (the fix-up
-// wave re-reads end states it has just written).
-__device__ __forceinline__ void drain() {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0xF70);
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
 )HIP";
 
 struct Gen {
@@ -245,6 +238,10 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "  extern __shared__ u32 s_dyn[];\n";
         s << "  u32* const s_bits = s_dyn;\n  u32& s_any = s_dyn[" << kFixBitmapWords << "];\n";
         s << "  u32 p_next = 1u, n_re = 0u, n_mis = 0u;\n  bool casc = false, use_bits = true;\n";
+        // end state of the last recomputed chunk, kept in registers for its
+        // successor (the walk is in increasing order, so no reload is needed)
+        s << "  u32 last_c = 0xfffffffeu;\n";
+        for (auto& kv : state_of) s << "  u64 st" << kv.second << " = 0ull;\n";
         s << "  if (fix) {\n";
         s << "    const bool fits = A.n_chunks <= 32u * " << kFixBitmapWords << "u;\n";
         s << "    const u32 nbw = fits ? (A.n_chunks + 31u) / 32u : 0u;\n";
@@ -279,7 +276,8 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     s << "  for (;;) {\n";
     if (has_state) {
         // mode 1, first wave: the next chunk to recompute, in order: a marked
-        // chunk, or the successor of a chunk whose end state changed
+        // chunk, or the successor of a recomputed chunk when its start state
+        // differs from the corrected end state
         s << "    if (fix) {\n";
         s << "      bool found = false;\n";
         s << "      while (!found) {\n";
@@ -304,10 +302,10 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "        const u32 ci = A.chunk_inst ? u32_at(A.chunk_inst, cand) : 0u;\n";
         s << "        bool need = false;\n";
         s << "        if (cand != inst_at(A.inst, ci).chunk_begin) {\n";
-        s << "          for (u32 i = lane; i < " << NS << "; i += 64u)\n";
-        s << "            if (aload64(&A.st_u[(u64)cand * " << NS << " + i]) != aload64(&A.st_t[(u64)(cand - 1u) * " << NS
-          << " + i])) need = true;\n";
-        s << "          need = ballot(need) != 0ull;\n";
+        s << "          const bool regs = cand == last_c + 1u;\n";
+        for (auto& kv : state_of)
+            s << "          need |= aload64(&A.st_u[(u64)cand * " << NS << " + " << kv.second << "u]) != (regs ? st"
+              << kv.second << " : aload64(&A.st_t[(u64)(cand - 1u) * " << NS << " + " << kv.second << "u]));\n";
         s << "        }\n";
         s << "        p_next = cand + 1u;\n";
         s << "        casc = false;\n";
@@ -325,9 +323,12 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     s << "    const u64 r_end = (r_begin + " << CR << "ull < D.rows) ? r_begin + " << CR << "ull : D.rows;\n";
     if (has_state) {
         s << "    const u64 warm = (!fix && cc > 0) ? (r_begin < " << WR << "ull ? r_begin : " << WR << "ull) : 0ull;\n";
+        s << "    if (!fix) {\n";
+        for (auto& kv : state_of) s << "      st" << kv.second << " = 0ull;\n";
+        s << "    } else if (c != last_c + 1u) {\n";
         for (auto& kv : state_of)
-            s << "    u64 st" << kv.second << " = fix ? aload64(&A.st_t[(u64)(c - 1u) * " << NS << " + " << kv.second
-              << "u]) : 0ull;\n";
+            s << "      st" << kv.second << " = aload64(&A.st_t[(u64)(c - 1u) * " << NS << " + " << kv.second << "u]);\n";
+        s << "    }\n";
         // the start state a recomputed chunk used
         s << "    if (fix) {\n";
         for (auto& kv : state_of)
@@ -486,24 +487,21 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "      }\n    }\n";
     }
     if (has_state) {
-        // end state T[c]; a recomputed chunk also learns whether it changed
+        // end state T[c]
         s << "    if (!fix) {\n";
         for (auto& kv : state_of)
             s << "      if (lane == " << (kv.second % 64) << ") A.st_t[(u64)c * " << NS << " + " << kv.second << "u] = st"
               << kv.second << ";\n";
         s << "      return;\n";
         s << "    }\n";
-        s << "    bool changed = false;\n";
-        for (auto& kv : state_of) {
-            s << "    if (lane == " << (kv.second % 64) << ") {\n";
-            s << "      u64* tp = &A.st_t[(u64)c * " << NS << " + " << kv.second << "u];\n";
-            s << "      if (aload64(tp) != st" << kv.second << ") changed = true;\n";
-            s << "      astore64(tp, st" << kv.second << ");\n";
-            s << "    }\n";
-        }
-        s << "    casc = ballot(changed) != 0ull;\n";
+        // the corrected end state stays in registers for the successor,
+        // which is checked next
+        for (auto& kv : state_of)
+            s << "    if (lane == " << (kv.second % 64) << ") astore64(&A.st_t[(u64)c * " << NS << " + " << kv.second
+              << "u], st" << kv.second << ");\n";
+        s << "    casc = true;\n";
+        s << "    last_c = c;\n";
         s << "    ++n_re;\n";
-        s << "    drain();\n";
         s << "  }\n";  // job loop
         s << "fix_done:\n";
         s << "  if (lane == 0) {\n";
