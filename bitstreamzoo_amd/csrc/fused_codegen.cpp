@@ -196,9 +196,13 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         opt.occupancy > 0 ? std::max(0, 512 / opt.occupancy - 24 - NPOP_ - live_) : 96;
     const int u_cap = std::max(1, std::min(opt.rows_per_step, load_budget / std::max(1, 2 * loads_per_row)));
     const int WR = has_state ? std::max(1, opt.warmup_rows) : 0;
-    // A step may cover a whole chunk including its warm-up rows, so that a
-    // short chunk is one round of loads; rows past the chunk are skipped.
-    const int U = std::min(u_cap, CR + WR);
+    // Rows per step divide the chunk length.  (Letting a step run past the
+    // chunk, e.g. 3 rows per step over 16-row chunks, gave wrong results on
+    // MI355X for a kernel with 97 state words, with no difference in the
+    // generated source's semantics found; that form is not generated.)
+    int U = 1;
+    for (int d = 1; d <= std::min(u_cap, CR); ++d)
+        if (CR % d == 0) U = d;
     ks.rows_per_step = U;
     ks.chunk_rows = CR;
 

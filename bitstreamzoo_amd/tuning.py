@@ -175,7 +175,7 @@ def default_space(kind: str = "auto") -> list[dict]:
         grid = {"chunk_rows": [16, 32], "rows_per_step": [1, 2], "occupancy": [1, 2, 4],
                 "waves_per_block": [2, 4]}
     elif kind == "stateful":
-        grid = {"chunk_rows": [4, 8, 16, 32], "rows_per_step": [4, 9, 17, 33], "occupancy": [0],
+        grid = {"chunk_rows": [4, 8, 16, 32], "rows_per_step": [2, 4, 8, 16], "occupancy": [0],
                 "waves_per_block": [1, 4]}
     elif kind == "small":
         grid = {"chunk_rows": [2, 4, 8], "rows_per_step": [2, 4, 8], "occupancy": [0, 4],

@@ -57,3 +57,31 @@ def test_dag_batch_of_uneven_texts():
             assert words_to_int(plan.download_output(i, name, nw)) == exp[name], (i, name)
     plan.close()
     res.free()
+
+
+ROWS_SWEEP = [(16, 2), (16, 3), (16, 4), (16, 5), (16, 6), (16, 7), (4, 3), (8, 3), (12, 3)]
+
+
+@pytest.mark.parametrize("chunk_rows,rows_per_step", ROWS_SWEEP)
+@pytest.mark.parametrize("occ", [1, 0])
+def test_rows_per_step_sweep_wide_kernel(chunk_rows, rows_per_step, occ):
+    """A wide partition (max_live 160: ~100 stateful ops per kernel) with
+    every rows-per-step count, including ones that do not divide the chunk."""
+    from bitstreamzoo_amd import schedsearch
+    spec = schedsearch.random_specs(5, 101, nodes=(2000, 5000), log2_bytes=(23, 25))[3]
+    p = dag_gen.generate(spec.nodes, seed=spec.seed)
+    case = dag_gen.text_case(64 * 64 * 40 + 123, seed=spec.seed + 7)
+    exp = oracle_outputs(p, case)
+    c = ctx()
+    res = tuning.Resident(c, [case])
+    opts = {"chunk_rows": chunk_rows, "rows_per_step": rows_per_step, "occupancy": occ, "max_live": 160,
+            "max_ops": 4000}
+    plan = api.Plan(c, api.Program(p.source), tuning.shape_case([case]), opts)
+    res.bind(plan)
+    plan.run()
+    nw = (case.bitlength + 63) // 64
+    rps = [k["rows_per_step"] for k in plan.info()["kernels"]]
+    bad = [n for n in p.outputs if words_to_int(plan.download_output(0, n, nw)) != exp[n]]
+    plan.close()
+    res.free()
+    assert not bad, f"{len(bad)} outputs differ; effective rows per step {rps[:4]}"
