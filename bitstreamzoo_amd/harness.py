@@ -1,7 +1,7 @@
 """Benchmark-harness driver for `bsim_hip`.
 
 `HipBackend` has the interface of the reference harness's backends
-(`Backend.run` / `run_batch`, benchmark/base.py:22-30 and the CudaBackend at
+(`Backend.run` / `run_batch`, benchmark/base.py:23-30 and the CudaBackend at
 benchmark/base.py:181-254): it runs the `bsim_hip` executable once per case
 (JSON on stdin) or per `.bsdata` file, and turns its JSON back into Python
 ints.  Registering it as backend "hip" in the reference's
