@@ -172,6 +172,91 @@ def evaluate(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSp
     return rows
 
 
+@dataclass
+class MixedResult:
+    n_programs: int
+    bit_ops: int
+    alg_bytes: int
+    ms: float
+    default_ms: float
+    pick_s: float
+    compile_s: float
+    judged: list = field(default_factory=list)   # (program index, model pick ms, autotuned best ms)
+
+    @property
+    def gbitops(self) -> float:
+        return self.bit_ops / (self.ms * 1e-3) / 1e9
+
+    @property
+    def gbps(self) -> float:
+        return self.alg_bytes / (self.ms * 1e-3) / 1e9
+
+
+def mixed_batch(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSpec], candidates: list[dict],
+                judge_frac: float = 0.05, steps: int = 10, processes: int = 16, seed: int = 0,
+                log=None) -> MixedResult:
+    """A batch of different generated programs run as one captured graph:
+    the model picks every program's schedule from plan features, all picks
+    are compiled in parallel, every program's outputs are checked against the
+    same program under the default schedule, the whole batch is replayed as
+    one hipGraph, and the measured autotuner re-tunes a random `judge_frac`
+    of the programs as judge."""
+    built = [sp.build() for sp in specs]
+    t0 = time.perf_counter()
+    picks = []
+    for (p, case) in built:
+        prog = api.Program(p.source)
+        F = costmodel.features_batch(prog, tuning.shape_case([case]), candidates, max(1, (case.bitlength + 63) // 64), 1)
+        picks.append(candidates[int(np.argmin(model.predict_ms(F)))])
+    pick_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    for (p, case), opt in zip(built, picks):
+        tuning.precompile(p.source, [case], [opt, {}], processes=processes)
+    compile_s = time.perf_counter() - t1
+    residents, plans, defaults = [], [], []
+    bit_ops = alg = 0
+    try:
+        for (p, case), opt in zip(built, picks):
+            r = tuning.Resident(ctx, [case])
+            residents.append(r)
+            prog = api.Program(p.source)
+            pl = api.Plan(ctx, prog, tuning.shape_case([case]), opt)
+            r.bind(pl)
+            plans.append(pl)
+            d = api.Plan(ctx, prog, tuning.shape_case([case]), {})
+            r.bind(d)
+            defaults.append(d)
+            bit_ops += p.n_dsl_ops * case.bitlength
+            nw = (case.bitlength + 63) // 64
+            alg += 8 * nw * (8 + len(p.outputs))
+        for pl, d, (p, case) in zip(plans, defaults, built):
+            pl.run()
+            d.run()
+            nw = (case.bitlength + 63) // 64
+            for o in p.outputs:
+                if not np.array_equal(pl.download_output(0, o, nw), d.download_output(0, o, nw)):
+                    raise RuntimeError(f"schedule {pl.info()['kernels'][:1]} changed output {o}")
+        ms = tuning.time_runs(ctx, plans, steps, 2) * len(plans)        # one batch = every program once
+        default_ms = tuning.time_runs(ctx, defaults, steps, 2) * len(defaults)
+    finally:
+        for pl in plans + defaults:
+            pl.close()
+        for r in residents:
+            r.free()
+    res = MixedResult(len(specs), bit_ops, alg, ms, default_ms, pick_s, compile_s)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n_judge = max(1, int(round(judge_frac * len(specs))))
+    for i in sorted(rng.choice(len(specs), size=n_judge, replace=False).tolist()):
+        p, case = built[i]
+        m = measure(ctx, p.source, case, candidates, processes=processes)
+        k = candidates.index(picks[i])
+        res.judged.append((i, float(m.ms[k]), float(min(m.ms))))
+        if log:
+            log(f"judge program {i} ({specs[i].nodes} nodes, {specs[i].n_bytes} B): model pick {m.ms[k]:.3f} ms, "
+                f"autotuned best {min(m.ms):.3f} ms")
+    return res
+
+
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
 
@@ -211,6 +296,7 @@ def main(argv=None) -> int:
     ap.add_argument("--min-log2-bytes", type=float, default=20)
     ap.add_argument("--max-log2-bytes", type=float, default=24)
     ap.add_argument("--collect-only", action="store_true")
+    ap.add_argument("--mixed", type=int, default=0, help="run a mixed batch of N programs instead of --eval")
     a = ap.parse_args(argv)
 
     def log(m):
@@ -236,6 +322,18 @@ def main(argv=None) -> int:
     loss = model.fit(X, y)
     log(f"trained on {len(y)} rows in {model.train_seconds:.2f} s, loss {loss:.4f}")
     cands = sample_space(a.candidates, 12345)
+    if a.mixed:
+        specs = random_specs(a.mixed, a.seed + 200, nodes=(500, 5000), log2_bytes=(16, 22))
+        r = mixed_batch(ctx, model, specs, cands, processes=a.processes, seed=a.seed, log=log)
+        print(json.dumps({"programs": r.n_programs, "ms_per_batch": round(r.ms, 4),
+                          "default_schedules_ms": round(r.default_ms, 4), "Gbit_ops_per_s": round(r.gbitops, 1),
+                          "GBps": round(r.gbps, 1), "hbm_frac": round(r.gbps / 8000.0, 4),
+                          "model_pick_s": round(r.pick_s, 2), "compile_s": round(r.compile_s, 1),
+                          "judged": [{"program": i, "pick_ms": round(a_, 4), "best_ms": round(b_, 4),
+                                      "gap_pct": round(100 * (a_ / b_ - 1), 2)} for i, a_, b_ in r.judged]}),
+              flush=True)
+        ctx.close()
+        return 0
     held = random_specs(a.eval, a.seed + 100, nodes=(2000, 5000), log2_bytes=(23, 25))
     rows = evaluate(ctx, model, held, cands, processes=a.processes, log=log)
     summary = {
