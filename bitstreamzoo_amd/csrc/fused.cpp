@@ -141,6 +141,7 @@ void mkdirs(const std::string& path) {
 // The entry points of a plan's kernels (code objects owned by the Jit).
 struct Module {
     std::vector<hipFunction_t> fns;
+    std::vector<size_t> scratch;   // private (spill) bytes per lane of each kernel
 };
 using ModuleP = std::shared_ptr<Module>;
 
@@ -214,6 +215,11 @@ public:
                 it = loaded_.emplace(h, std::make_pair(mod, f)).first;
             }
             m->fns.push_back(it->second.second);
+            int local = 0;
+            m->scratch.push_back(
+                hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, it->second.second) == hipSuccess
+                    ? (size_t)std::max(0, local)
+                    : 0);
         }
         return m;
     }
@@ -1231,7 +1237,9 @@ std::string FusedPlan::info_json() const {
         o << (k ? "," : "") << "{\"name\":\"" << ks.name << "\",\"nodes\":" << ks.nodes.size()
           << ",\"inputs\":" << ks.inputs.size() << ",\"outputs\":" << ks.outputs.size() << ",\"pops\":" << ks.pops.size()
           << ",\"state_words\":" << ks.n_state << ",\"rows_per_step\":" << ks.rows_per_step
-          << ",\"occupancy\":" << ks.occupancy << "}";
+          << ",\"occupancy\":" << ks.occupancy;
+        if (I.mod && k < I.mod->scratch.size()) o << ",\"scratch_bytes\":" << I.mod->scratch[k];
+        o << "}";
     }
     o << "],\"sources\":[";
     for (size_t j = 0; j < I.sources.size(); ++j)

@@ -564,13 +564,24 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
         if (arity(x.op) >= 1) last_use[x.a] = std::max(last_use[x.a], n);
         if (arity(x.op) >= 2) last_use[x.b] = std::max(last_use[x.b], n);
     }
+    // register liveness: outputs and popcounts are stored / counted right
+    // after their producer, so only consumer nodes keep a value in registers
+    const std::vector<int32_t> node_last_use = last_use;
     for (int32_t o : g.outputs) last_use[o] = N;
     for (int32_t p : g.pops) last_use[p] = N;
 
     // ── greedy linear partitioning ──
+    // A kernel holds, per row, its live in-kernel values and every word it
+    // loads (each distinct external operand, and each shift of one); all of
+    // them count against max_live, which is also capped by the register file
+    // at the requested occupancy, so a generated kernel does not spill.
+    const int occ_floor = opt.occupancy > 0 ? opt.occupancy : 1;
+    const int value_budget = std::max(8, std::min(opt.max_live, (512 / occ_floor - 48) / 2));
     std::vector<int32_t> kern_of(N, -1);
     std::vector<std::vector<int32_t>> kernels;
+    std::set<std::pair<int32_t, uint64_t>> loaded;  // (external operand, shift or 0) words of this kernel
     int live = 0, n_state = 0;
+    auto is_const = [&](int32_t a) { return g.nodes[a].op == NOp::Zero || g.nodes[a].op == NOp::Ones; };
     for (int32_t n = 0; n < N; ++n) {
         if (!needed[n]) continue;
         const GNode& x = g.nodes[n];
@@ -581,22 +592,32 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
             auto inside = [&](int32_t a) { return a >= 0 && kern_of[a] == K; };
             if (needs_external_operand(x) && inside(x.a)) cut = true;
             if ((int)kernels.back().size() >= opt.max_ops) cut = true;
-            int delta = last_use[n] > n ? 1 : 0;
-            if (arity(x.op) >= 1 && inside(x.a) && last_use[x.a] == n) --delta;
-            if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && last_use[x.b] == n) --delta;
-            if (live + delta > opt.max_live) cut = true;
+            int delta = node_last_use[n] > n ? 1 : 0;
+            if (arity(x.op) >= 1 && inside(x.a) && node_last_use[x.a] == n) --delta;
+            if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && node_last_use[x.b] == n) --delta;
+            int new_loads = 0;
+            const uint64_t sk = (x.op == NOp::Shl || x.op == NOp::Shr) ? (x.op == NOp::Shr ? (1ull << 63) : 0) | x.k : 0;
+            if (arity(x.op) >= 1 && !inside(x.a) && !is_const(x.a) && !loaded.count({x.a, sk})) ++new_loads;
+            if (arity(x.op) >= 2 && !inside(x.b) && !is_const(x.b) && x.b != x.a && !loaded.count({x.b, 0})) ++new_loads;
+            if (live + delta + (int)loaded.size() + new_loads > value_budget) cut = true;
             if (n_state >= 512 && (x.op == NOp::Add || (stateful_shift(x) && inside(x.a)))) cut = true;
         }
         if (cut) {
             kernels.emplace_back();
             live = 0;
             n_state = 0;
+            loaded.clear();
         }
         const int KK = (int)kernels.size() - 1;
         auto inside = [&](int32_t a) { return a >= 0 && kern_of[a] == KK; };
-        if (arity(x.op) >= 1 && inside(x.a) && last_use[x.a] == n) --live;
-        if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && last_use[x.b] == n) --live;
-        if (last_use[n] > n) ++live;
+        {
+            const uint64_t sk = (x.op == NOp::Shl || x.op == NOp::Shr) ? (x.op == NOp::Shr ? (1ull << 63) : 0) | x.k : 0;
+            if (arity(x.op) >= 1 && !inside(x.a) && !is_const(x.a)) loaded.insert({x.a, sk});
+            if (arity(x.op) >= 2 && !inside(x.b) && !is_const(x.b)) loaded.insert({x.b, 0});
+        }
+        if (arity(x.op) >= 1 && inside(x.a) && node_last_use[x.a] == n) --live;
+        if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && node_last_use[x.b] == n) --live;
+        if (node_last_use[n] > n) ++live;
         if (x.op == NOp::Add || (stateful_shift(x) && inside(x.a))) ++n_state;
         kern_of[n] = KK;
         kernels.back().push_back(n);

@@ -85,3 +85,34 @@ def test_rows_per_step_sweep_wide_kernel(chunk_rows, rows_per_step, occ):
     plan.close()
     res.free()
     assert not bad, f"{len(bad)} outputs differ; effective rows per step {rps[:4]}"
+
+
+WIDE_REGRESSIONS = [
+    # (spec list args, index, schedule): wide partitions that once gave wrong
+    # outputs when their kernels exhausted the register file and spilled
+    (dict(n=32, seed=201, nodes=(500, 5000), log2_bytes=(16, 22)), 25,
+     {"chunk_rows": 4, "max_live": 160, "max_ops": 1000, "occupancy": 1, "rows_per_step": 1}),
+    (dict(n=5, seed=101, nodes=(2000, 5000), log2_bytes=(23, 25)), 3,
+     {"chunk_rows": 16, "max_live": 160, "max_ops": 4000, "occupancy": 0, "rows_per_step": 4}),
+]
+
+
+@pytest.mark.parametrize("args,idx,sched", WIDE_REGRESSIONS)
+def test_wide_schedules_match_oracle_without_spills(args, idx, sched):
+    from bitstreamzoo_amd import schedsearch
+    sp = schedsearch.random_specs(args["n"], args["seed"], nodes=args["nodes"], log2_bytes=args["log2_bytes"])[idx]
+    p = dag_gen.generate(sp.nodes, seed=sp.seed)
+    case = dag_gen.text_case(min(sp.n_bytes, 400_000), seed=sp.seed + 7)
+    exp = oracle_outputs(p, case)
+    c = ctx()
+    res = tuning.Resident(c, [case])
+    plan = api.Plan(c, api.Program(p.source), tuning.shape_case([case]), sched)
+    res.bind(plan)
+    plan.run()
+    nw = (case.bitlength + 63) // 64
+    ks = plan.info()["kernels"]
+    bad = [n for n in p.outputs if words_to_int(plan.download_output(0, n, nw)) != exp[n]]
+    plan.close()
+    res.free()
+    assert all(k.get("scratch_bytes", 0) == 0 for k in ks), [k.get("scratch_bytes") for k in ks]
+    assert not bad, f"{len(bad)} outputs differ"
