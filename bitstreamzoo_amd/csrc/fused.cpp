@@ -1222,7 +1222,38 @@ std::string FusedPlan::describe_json(const std::string& options_array) const {
               << ",\"state_words\":" << ks.n_state << ",\"rows_per_step\":" << ks.rows_per_step
               << ",\"occupancy\":" << ks.occupancy << "}";
         }
-        o << "],\"slots\":" << ps.n_slots << ",\"chunk_rows\":" << ps.opt.chunk_rows << "}";
+        o << "],\"slots\":" << ps.n_slots << ",\"chunk_rows\":" << ps.opt.chunk_rows;
+        if (const Json* d = arr.a[i].get("detail"); d && d->as_int(0)) {
+            // the whole plan, for checking a partition on the host
+            const SegGraph& g = impl_->spec->g;
+            o << ",\"graph\":[";
+            for (size_t n = 0; n < g.nodes.size(); ++n)
+                o << (n ? "," : "") << "[" << (int)g.nodes[n].op << "," << g.nodes[n].a << "," << g.nodes[n].b << ","
+                  << g.nodes[n].k << "," << g.nodes[n].src << "]";
+            o << "],\"graph_outputs\":[";
+            for (size_t n = 0; n < g.outputs.size(); ++n) o << (n ? "," : "") << g.outputs[n];
+            o << "],\"named_outputs\":[";
+            for (size_t n = 0; n < impl_->outs.size(); ++n)
+                o << (n ? "," : "") << "[\"" << impl_->outs[n].name << "\"," << impl_->outs[n].ref << "]";
+            o << "],\"sources\":[";
+            for (size_t n = 0; n < impl_->sources.size(); ++n)
+                o << (n ? "," : "") << "[\"" << impl_->sources[n].first << "\"," << impl_->sources[n].second << "]";
+            o << "],\"kernel_detail\":[";
+            for (size_t k = 0; k < ps.kernels.size(); ++k) {
+                const KernelSpec& ks = ps.kernels[k];
+                o << (k ? "," : "") << "{\"nodes\":[";
+                for (size_t j = 0; j < ks.nodes.size(); ++j) o << (j ? "," : "") << ks.nodes[j];
+                o << "],\"inputs\":[";
+                for (size_t j = 0; j < ks.inputs.size(); ++j)
+                    o << (j ? "," : "") << "[" << (int)ks.inputs[j].kind << "," << ks.inputs[j].id << "," << ks.input_node[j] << "]";
+                o << "],\"outputs\":[";
+                for (size_t j = 0; j < ks.outputs.size(); ++j)
+                    o << (j ? "," : "") << "[" << ks.outputs[j] << "," << ks.out_slot[j] << "]";
+                o << "]}";
+            }
+            o << "]";
+        }
+        o << "}";
     }
     o << "]";
     return o.str();

@@ -129,6 +129,14 @@ std::string hex64(uint64_t v) {
     return b;
 }
 
+// Registers a generated kernel may use: the 256 architectural VGPRs at most
+// (kernels that overflowed into AGPRs gave wrong results on gfx950), fewer at
+// a higher requested occupancy.
+int kernel_registers(const PlanOptions& opt) {
+    const int occ = opt.occupancy > 0 ? opt.occupancy : 1;
+    return std::min(256, 512 / occ);
+}
+
 // Generates one kernel; fills ks.source and ks.n_state.
 void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool emit) {
     std::set<int32_t> in_kernel(ks.nodes.begin(), ks.nodes.end());
@@ -189,11 +197,11 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     // The chunk length is the plan's, shared by all its kernels.
     const int loads_per_row = (int)(plain_inputs.size() + off_words.size());
     const int CR = std::max(1, opt.chunk_rows);
-    // With an explicit occupancy the prefetch may fill that register budget.
+    // Prefetched words (2 VGPRs each, per row in flight) share the kernel's
+    // register file with its live values.
     const int NPOP_ = (int)ks.pops.size();
-    const int live_ = std::min(opt.max_live, (int)ks.nodes.size());
-    const int load_budget =
-        opt.occupancy > 0 ? std::max(0, 512 / opt.occupancy - 24 - NPOP_ - live_) : 96;
+    const int live_ = ks.peak_live;
+    const int load_budget = std::max(0, kernel_registers(opt) - 48 - NPOP_ - 2 * live_);
     const int u_cap = std::max(1, std::min(opt.rows_per_step, load_budget / std::max(1, 2 * loads_per_row)));
     const int WR = has_state ? std::max(1, opt.warmup_rows) : 0;
     // Rows per step divide the chunk length.  (Letting a step run past the
@@ -211,7 +219,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     // Register budget: at least `occupancy` waves per SIMD (512/occupancy VGPRs),
     // which also bounds how many row loads the scheduler hoists.
     // estimated VGPRs: prefetched words, popcount accumulators, live values
-    const int est_vgprs = 2 * loads_per_row * U + NPOP + 2 * std::min(opt.max_live, (int)ks.nodes.size()) / 2 + 24;
+    const int est_vgprs = 2 * loads_per_row * U + NPOP + 2 * live_ + 24;
     int occ = opt.occupancy;
     if (occ <= 0) {
         occ = 4;
@@ -575,10 +583,10 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
     // loads (each distinct external operand, and each shift of one); all of
     // them count against max_live, which is also capped by the register file
     // at the requested occupancy, so a generated kernel does not spill.
-    const int occ_floor = opt.occupancy > 0 ? opt.occupancy : 1;
-    const int value_budget = std::max(8, std::min(opt.max_live, (512 / occ_floor - 48) / 2));
+    const int value_budget = std::max(8, std::min(opt.max_live, (kernel_registers(opt) - 64) / 2));
     std::vector<int32_t> kern_of(N, -1);
     std::vector<std::vector<int32_t>> kernels;
+    std::vector<int> peak_live;                    // per kernel: most in-kernel values live at once
     std::set<std::pair<int32_t, uint64_t>> loaded;  // (external operand, shift or 0) words of this kernel
     int live = 0, n_state = 0;
     auto is_const = [&](int32_t a) { return g.nodes[a].op == NOp::Zero || g.nodes[a].op == NOp::Ones; };
@@ -604,6 +612,7 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
         }
         if (cut) {
             kernels.emplace_back();
+            peak_live.push_back(0);
             live = 0;
             n_state = 0;
             loaded.clear();
@@ -618,6 +627,7 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
         if (arity(x.op) >= 1 && inside(x.a) && node_last_use[x.a] == n) --live;
         if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && node_last_use[x.b] == n) --live;
         if (node_last_use[n] > n) ++live;
+        peak_live.back() = std::max(peak_live.back(), live);
         if (x.op == NOp::Add || (stateful_shift(x) && inside(x.a))) ++n_state;
         kern_of[n] = KK;
         kernels.back().push_back(n);
@@ -635,6 +645,7 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
         std::snprintf(nm, sizeof nm, "bsz_k%d", k);
         P.kernels[k].name = nm;
         P.kernels[k].nodes = kernels[k];
+        P.kernels[k].peak_live = k < (int)peak_live.size() ? peak_live[k] : 0;
     }
 
     // consumers per kernel, for materialization and slot intervals

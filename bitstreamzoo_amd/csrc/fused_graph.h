@@ -84,6 +84,7 @@ struct KernelSpec {
     std::vector<int32_t> out_slot;   // per output: -1 segment output, else intermediate slot
     std::vector<int32_t> pops;       // index into SegGraph::pops accumulated here
     int n_state = 0;                 // 64-bit state words
+    int peak_live = 0;               // most in-kernel values live at once (partitioner's count)
     int rows_per_step = 1;           // as generated (capped by register budget)
     int chunk_rows = 16;             // as generated
     int occupancy = 4;               // waves per SIMD the register budget was set for
