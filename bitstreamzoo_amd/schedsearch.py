@@ -30,7 +30,7 @@ import numpy as np
 
 from . import api, costmodel, dag_gen, tuning
 
-SPACE_GRID = {"max_live": [32, 64, 96, 160], "max_ops": [250, 1000, 4000], "chunk_rows": [4, 16, 64],
+SPACE_GRID = {"max_live": [24, 48, 72, 96], "max_ops": [250, 1000, 4000], "chunk_rows": [4, 16, 64],
               "rows_per_step": [1, 4], "occupancy": [0, 1]}
 
 
@@ -103,8 +103,9 @@ def measure(ctx: api.Context, source: str, case, candidates: list[dict], steps: 
 
 
 def collect(ctx: api.Context, specs: list[ProgramSpec], per_program: int = 16, seed: int = 0, processes: int = 16,
-            log=None):
-    """Training rows (features, ms, program index) from the measured tuner."""
+            log=None, out: str = ""):
+    """Training rows (features, ms, program index) from the measured tuner;
+    with `out`, the rows so far are saved after every program."""
     X, y, gid = [], [], []
     for i, sp in enumerate(specs):
         p, case = sp.build()
@@ -113,6 +114,9 @@ def collect(ctx: api.Context, specs: list[ProgramSpec], per_program: int = 16, s
         X += m.features
         y += m.ms
         gid += [i] * len(m.ms)
+        if out:
+            np.savez(out, X=np.array(X, dtype=np.float32), ms=np.array(y), gid=np.array(gid),
+                     features=np.array(costmodel.FEATURES))
         if log:
             log(f"collect {i + 1}/{len(specs)}: {sp.nodes} nodes, {sp.n_bytes} bytes, best {min(m.ms):.3f} ms, "
                 f"worst {max(m.ms):.3f} ms (compile {m.compile_s:.1f} s, measure {m.measure_s:.1f} s)")
@@ -312,7 +316,7 @@ def main(argv=None) -> int:
     else:
         specs = random_specs(a.programs, a.seed, nodes=(a.min_nodes, a.max_nodes),
                              log2_bytes=(a.min_log2_bytes, a.max_log2_bytes))
-        X, y, gid = collect(ctx, specs, a.per_program, seed=a.seed, processes=a.processes, log=log)
+        X, y, gid = collect(ctx, specs, a.per_program, seed=a.seed, processes=a.processes, log=log, out=a.out)
         if a.out:
             np.savez(a.out, X=X, ms=y, gid=gid, features=np.array(costmodel.FEATURES))
         if a.collect_only:
