@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 #include <set>
@@ -42,6 +43,9 @@ FusedOptions FusedOptions::from_env() {
     o.warmup_rows = env_int("BSZ_WARMUP_ROWS", o.warmup_rows);
     o.waves_per_block = env_int("BSZ_WAVES_PER_BLOCK", o.waves_per_block);
     o.rows_per_step = env_int("BSZ_ROWS_PER_STEP", o.rows_per_step);
+    o.words_per_lane = env_int("BSZ_WORDS_PER_LANE", o.words_per_lane);
+    o.store_order = env_int("BSZ_STORE_ORDER", o.store_order);
+    o.nontemporal = env_int("BSZ_NONTEMPORAL", o.nontemporal);
     o.max_ops = env_int("BSZ_MAX_OPS", o.max_ops);
     o.max_live = env_int("BSZ_MAX_LIVE", o.max_live);
     o.occupancy = env_int("BSZ_OCCUPANCY", o.occupancy);
@@ -64,6 +68,9 @@ FusedOptions FusedOptions::from_json_obj(const Json& j) {
     get("warmup_rows", o.warmup_rows);
     get("waves_per_block", o.waves_per_block);
     get("rows_per_step", o.rows_per_step);
+    get("words_per_lane", o.words_per_lane);
+    get("store_order", o.store_order);
+    get("nontemporal", o.nontemporal);
     get("max_ops", o.max_ops);
     get("max_live", o.max_live);
     get("occupancy", o.occupancy);
@@ -80,6 +87,9 @@ PlanOptions to_plan_options(const FusedOptions& f) {
     p.warmup_rows = std::max(1, f.warmup_rows);
     p.waves_per_block = std::min(16, std::max(1, f.waves_per_block));
     p.rows_per_step = std::max(1, f.rows_per_step);
+    p.words_per_lane = f.words_per_lane >= 2 ? 2 : 1;
+    p.store_order = f.store_order != 0;
+    p.nontemporal = std::max(0, std::min(2, f.nontemporal));
     p.max_ops = std::max(1, f.max_ops);
     p.max_live = std::max(4, f.max_live);
     p.occupancy = f.occupancy <= 0 ? 0 : std::min(8, f.occupancy);
@@ -138,10 +148,49 @@ void mkdirs(const std::string& path) {
     mkdir(path.c_str(), 0755);
 }
 
+// Register use of a compiled kernel, read from its code object's metadata
+// (the msgpack note of an AMDGPU code object: ".vgpr_count", ".agpr_count",
+// ".private_segment_fixed_size").
+struct KernelRegs {
+    int vgprs = -1, agprs = -1, scratch = -1;
+};
+
+int64_t msgpack_uint_after(const std::vector<char>& code, const std::string& key) {
+    // a msgpack fixstr key (length < 32) followed by a non-negative integer
+    std::string pat;
+    pat.push_back((char)(0xa0 | (key.size() & 31)));
+    pat += key;
+    auto it = std::search(code.begin(), code.end(), pat.begin(), pat.end());
+    if (it == code.end()) return -1;
+    size_t i = (size_t)(it - code.begin()) + pat.size();
+    if (i >= code.size()) return -1;
+    const uint8_t t = (uint8_t)code[i];
+    auto be = [&](size_t n) -> int64_t {
+        if (i + n >= code.size()) return -1;
+        uint64_t v = 0;
+        for (size_t k = 1; k <= n; ++k) v = (v << 8) | (uint8_t)code[i + k];
+        return (int64_t)v;
+    };
+    if (t < 0x80) return t;
+    if (t == 0xcc) return be(1);
+    if (t == 0xcd) return be(2);
+    if (t == 0xce) return be(4);
+    return -1;
+}
+
+KernelRegs kernel_regs(const std::vector<char>& code) {
+    KernelRegs r;
+    r.vgprs = (int)msgpack_uint_after(code, ".vgpr_count");
+    r.agprs = (int)msgpack_uint_after(code, ".agpr_count");
+    r.scratch = (int)msgpack_uint_after(code, ".private_segment_fixed_size");
+    return r;
+}
+
 // The entry points of a plan's kernels (code objects owned by the Jit).
 struct Module {
     std::vector<hipFunction_t> fns;
     std::vector<size_t> scratch;   // private (spill) bytes per lane of each kernel
+    std::vector<KernelRegs> regs;  // from the code-object metadata
 };
 using ModuleP = std::shared_ptr<Module>;
 
@@ -198,6 +247,18 @@ public:
         if (!err.empty()) throw RunError(err);
     }
 
+    // Register use of each kernel of `p` from its compiled code object
+    // (vgprs -1 where the unit has not been compiled).
+    std::vector<KernelRegs> regs_of(const PlanSpec& p) {
+        const std::string arch = default_arch();
+        std::vector<KernelRegs> out;
+        for (const KernelSpec& k : p.kernels) {
+            const std::vector<char> code = read_code(unit_hash(k.unit, arch));
+            out.push_back(code.empty() ? KernelRegs{} : kernel_regs(code));
+        }
+        return out;
+    }
+
     ModuleP load(const PlanSpec& p, double* compile_ms) {
         compile_all(p, compile_ms);
         const std::string arch = default_arch();
@@ -213,8 +274,10 @@ public:
                 hipFunction_t f;
                 BSZ_HIP(hipModuleGetFunction(&f, mod, "bsz_kernel"));
                 it = loaded_.emplace(h, std::make_pair(mod, f)).first;
+                regs_[h] = kernel_regs(code);
             }
             m->fns.push_back(it->second.second);
+            m->regs.push_back(regs_[h]);
             int local = 0;
             m->scratch.push_back(
                 hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, it->second.second) == hipSuccess
@@ -229,6 +292,7 @@ private:
     std::mutex mu_;
     std::unordered_map<uint64_t, std::vector<char>> codes_;                        // compiled this process
     std::unordered_map<uint64_t, std::pair<hipModule_t, hipFunction_t>> loaded_;    // on the device
+    std::unordered_map<uint64_t, KernelRegs> regs_;
 
     static uint64_t unit_hash(const std::string& unit, const std::string& arch) { return source_hash(unit + "|" + arch); }
     std::string path_of(uint64_t h) const {
@@ -875,6 +939,7 @@ struct FusedPlan::Impl {
     std::unique_ptr<FusedEngine> rec;       // the recording engine (owns the JIT and the pool)
     std::shared_ptr<PlanSpec> spec;
     ModuleP mod;
+    std::vector<KernelRegs> compiled_regs;                  // from the code objects, before any load
     std::unique_ptr<PlanRT> rt;
     std::vector<std::pair<std::string, int64_t>> sources;   // per source: (name, idx)
     struct Out { std::string name; int64_t idx; int kind; int32_t ref; };  // kind 0 node, 1 source alias, 2 const
@@ -1007,9 +1072,11 @@ std::unique_ptr<FusedPlan> FusedPlan::record(hipStream_t s, const Program& prog,
         I.ints.push_back(x);
     }
     I.spec = std::make_shared<PlanSpec>(build_plan(g, to_plan_options(opt)));
-    if (compile && !I.spec->kernels.empty())
+    if (compile && !I.spec->kernels.empty()) {
         I.rec->jit().compile_all(*I.spec, &I.rec->compile_ms(), std::max(0, opt.compile_shard),
                                  std::max(1, opt.compile_shards));
+        I.compiled_regs = I.rec->jit().regs_of(*I.spec);
+    }
     return fp;
 }
 
@@ -1220,7 +1287,7 @@ std::string FusedPlan::describe_json(const std::string& options_array) const {
             o << (k ? "," : "") << "{\"nodes\":" << ks.nodes.size() << ",\"inputs\":" << ks.inputs.size()
               << ",\"outputs\":" << ks.outputs.size() << ",\"pops\":" << ks.pops.size()
               << ",\"state_words\":" << ks.n_state << ",\"rows_per_step\":" << ks.rows_per_step
-              << ",\"occupancy\":" << ks.occupancy << "}";
+              << ",\"occupancy\":" << ks.occupancy << ",\"words_per_lane\":" << ks.words_per_lane << "}";
         }
         o << "],\"slots\":" << ps.n_slots << ",\"chunk_rows\":" << ps.opt.chunk_rows;
         if (const Json* d = arr.a[i].get("detail"); d && d->as_int(0)) {
@@ -1268,8 +1335,16 @@ std::string FusedPlan::info_json() const {
         o << (k ? "," : "") << "{\"name\":\"" << ks.name << "\",\"nodes\":" << ks.nodes.size()
           << ",\"inputs\":" << ks.inputs.size() << ",\"outputs\":" << ks.outputs.size() << ",\"pops\":" << ks.pops.size()
           << ",\"state_words\":" << ks.n_state << ",\"rows_per_step\":" << ks.rows_per_step
-          << ",\"occupancy\":" << ks.occupancy;
+          << ",\"occupancy\":" << ks.occupancy << ",\"words_per_lane\":" << ks.words_per_lane;
         if (I.mod && k < I.mod->scratch.size()) o << ",\"scratch_bytes\":" << I.mod->scratch[k];
+        if (I.mod && k < I.mod->regs.size()) {
+            o << ",\"vgprs\":" << I.mod->regs[k].vgprs << ",\"agprs\":" << I.mod->regs[k].agprs;
+        } else if (k < I.compiled_regs.size() && I.compiled_regs[k].vgprs >= 0) {
+            // compiled here but not loaded: the metadata's private segment
+            // size is the per-lane scratch the loader would report
+            const KernelRegs& r = I.compiled_regs[k];
+            o << ",\"scratch_bytes\":" << r.scratch << ",\"vgprs\":" << r.vgprs << ",\"agprs\":" << r.agprs;
+        }
         o << "}";
     }
     o << "],\"sources\":[";

@@ -92,12 +92,45 @@ __device__ __forceinline__ GAS u64* urow_w(u64* p) {
   const u64 v = (u64)p;
   return (GAS u64*)(((u64)uni((u32)(v >> 32)) << 32) | uni((u32)v));
 }
+// Streaming hints (set per unit): 1 = stores bypass the caches' normal
+// retention (nontemporal), 2 = loads too.
+#ifndef BSZ_NT
+#define BSZ_NT 0
+#endif
 // rowbase[lane] as scalar base + 32-bit byte offset (the saddr form)
 __device__ __forceinline__ u64 lane_ld(const GAS u64* rowbase, u32 off8) {
-  return *(const GAS u64*)((const GAS char*)rowbase + off8);
+  const GAS u64* p = (const GAS u64*)((const GAS char*)rowbase + off8);
+#if BSZ_NT >= 2
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
 }
 __device__ __forceinline__ void lane_st(GAS u64* rowbase, u32 off8, u64 v) {
-  *(GAS u64*)((GAS char*)rowbase + off8) = v;
+  GAS u64* p = (GAS u64*)((GAS char*)rowbase + off8);
+#if BSZ_NT >= 1
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+typedef u64 u64x2 __attribute__((ext_vector_type(2)));
+// words 2l, 2l+1 of a row pair: 16-byte load / store at row base + lane * 16
+__device__ __forceinline__ u64x2 lane_ld16(const GAS u64* rowbase, u32 off16) {
+  const GAS u64x2* p = (const GAS u64x2*)((const GAS char*)rowbase + off16);
+#if BSZ_NT >= 2
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ void lane_st16(GAS u64* rowbase, u32 off16, u64x2 v) {
+  GAS u64x2* p = (GAS u64x2*)((GAS char*)rowbase + off16);
+#if BSZ_NT >= 1
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
 }
 // word `w` of a stream of n words (0 outside), read at row base + per-lane offset
 __device__ __forceinline__ u64 ldr(const GAS u64* rowbase, long long off, long long w, u64 n) {
@@ -201,7 +234,11 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     // register file with its live values.
     const int NPOP_ = (int)ks.pops.size();
     const int live_ = ks.peak_live;
-    const int load_budget = std::max(0, kernel_registers(opt) - 48 - NPOP_ - 2 * live_);
+    // With an explicit occupancy the prefetch may use that occupancy's whole
+    // register file (the compiled kernel's register use is checked by the
+    // autotuner); by default it stays within 256 registers.
+    const int regs_for_loads = opt.occupancy > 0 ? 512 / opt.occupancy : 256;
+    const int load_budget = std::max(0, regs_for_loads - 48 - NPOP_ - 2 * live_);
     const int u_cap = std::max(1, std::min(opt.rows_per_step, load_budget / std::max(1, 2 * loads_per_row)));
     const int WR = has_state ? std::max(1, opt.warmup_rows) : 0;
     // Rows per step divide the chunk length.  (Letting a step run past the
@@ -211,7 +248,21 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     int U = 1;
     for (int d = 1; d <= std::min(u_cap, CR); ++d)
         if (CR % d == 0) U = d;
-    ks.rows_per_step = U;
+    // Two words per lane (16-byte loads and stores) where nothing crosses
+    // lanes: no carries, no in-kernel advances, no loads at a shifted offset.
+    // A lane then owns words 2l, 2l+1 of a pair of rows.
+    // Only where a row pair's loads (4 VGPRs each) fit the register budget.
+    const bool wide = opt.words_per_lane >= 2 && !has_state && ext_shifts.empty() && CR % 2 == 0 &&
+                      4 * loads_per_row + 2 * live_ <= kernel_registers(opt) - 48 - NPOP_;
+    if (wide) {
+        const int pairs = CR / 2;
+        const int cap2 = std::max(1, std::min(opt.rows_per_step, load_budget / std::max(1, 4 * loads_per_row)));
+        U = 1;
+        for (int d = 1; d <= std::min(cap2, pairs); ++d)
+            if (pairs % d == 0) U = d;
+    }
+    ks.words_per_lane = wide ? 2 : 1;
+    ks.rows_per_step = wide ? 2 * U : U;
     ks.chunk_rows = CR;
 
     std::ostringstream s;
@@ -219,17 +270,117 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     // Register budget: at least `occupancy` waves per SIMD (512/occupancy VGPRs),
     // which also bounds how many row loads the scheduler hoists.
     // estimated VGPRs: prefetched words, popcount accumulators, live values
-    const int est_vgprs = 2 * loads_per_row * U + NPOP + 2 * live_ + 24;
+    const int est_vgprs = (wide ? 4 : 2) * loads_per_row * U + NPOP + (wide ? 4 : 2) * live_ + 24;
+    // Automatic occupancy never goes below 2 waves per SIMD (256 registers),
+    // so an untuned kernel cannot spill into AGPRs; occupancy 1 is for the
+    // autotuner, which checks the compiled register use.
     int occ = opt.occupancy;
     if (occ <= 0) {
         occ = 4;
-        while (occ > 1 && est_vgprs > 512 / occ) --occ;
+        while (occ > 2 && est_vgprs > 512 / occ) --occ;
     }
     ks.occupancy = occ;
     const int WPB = opt.waves_per_block;
     const int LB = has_state ? std::max(64 * WPB, kFixThreads) : 64 * WPB;
     ks.launch_bounds = LB;
     if (!emit) return;  // shape only (register budget, rows per step), no source
+    if (wide) {
+        s << "extern \"C\" __global__ __launch_bounds__(" << LB << ") __attribute__((amdgpu_waves_per_eu(" << occ
+          << "))) void bsz_kernel(KArgs A) {\n";
+        s << "  const u32 lane = lane_id();\n";
+        s << "  const u32 off16 = lane * 16u;\n";
+        s << "  const u64x2 zero2 = {0ull, 0ull};\n";
+        if (NPOP) {
+            s << "  const u32 ep = u32_at(A.ctl, 0u) & 1u;\n";
+            s << "  u64* const bank = A.pop_acc + (ep ? A.pop_stride : 0ull);\n";
+        }
+        s << "  const u32 c = blockIdx.x * " << WPB << "u + uni(threadIdx.x >> 6);\n";
+        s << "  if (c >= A.n_chunks) return;\n";
+        if (NPOP) {
+            s << "  {\n    u64* const other = A.pop_acc + (ep ? 0ull : A.pop_stride);\n";
+            s << "    const u64 n_slots = ((u64)A.n_inst * " << NPOP << "ull) << A.spread_log;\n";
+            s << "    for (u64 i = (u64)c * 64ull + lane; i < n_slots; i += (u64)A.n_chunks * 64ull) other[i] = 0ull;\n  }\n";
+        }
+        s << "  const u32 ii = A.chunk_inst ? u32_at(A.chunk_inst, c) : 0u;\n";
+        s << "  const InstDesc D = inst_at(A.inst, ii);\n";
+        s << "  const u32 cc = c - D.chunk_begin;\n";
+        s << "  const u64 NW = D.rows * 64ull;\n";
+        for (int j = 0; j < NIN; ++j) s << "  const u64* __restrict__ in" << j << " = tab_in(A.in, (u64)ii * " << NIN << "u + " << j << "u);\n";
+        for (int j = 0; j < NOUT; ++j) s << "  u64* __restrict__ out" << j << " = tab_out(A.out, (u64)ii * " << NOUT << "u + " << j << "u);\n";
+        s << "  const u64 r_begin = (u64)cc * " << CR << "ull;\n";
+        s << "  const u64 r_end = (r_begin + " << CR << "ull < D.rows) ? r_begin + " << CR << "ull : D.rows;\n";
+        for (int p = 0; p < NPOP; ++p) s << "  u32 pc" << p << " = 0u;\n";
+        s << "  for (u64 r = r_begin; r < r_end; r += " << 2 * U << "ull) {\n";
+        for (int u = 0; u < U; ++u) {
+            // rows ru, ru+1; the second may not exist at the end of a stream
+            s << "    const u64 ru" << u << " = (u64)uni((u32)(r + " << 2 * u << "ull));\n";
+            s << "    const long long w" << u << " = (long long)(ru" << u << " * 64ull + 2u * lane);\n";
+            s << "    const bool ok" << u << " = r + " << 2 * u << "ull < r_end;\n";
+            s << "    const bool lv" << u << " = ok" << u << " && (u64)w" << u << " < NW;\n";
+            for (int32_t a : plain_inputs)
+                s << "    const u64x2 x" << a << "_" << u << " = lv" << u << " ? lane_ld16(urow(in" << input_index.at(a)
+                  << " + ru" << u << " * 64ull), off16) : zero2;\n";
+        }
+        std::map<int32_t, std::vector<int>> out_of, pop_of;
+        for (int j = 0; j < NOUT; ++j) out_of[ks.outputs[j]].push_back(j);
+        for (int p = 0; p < NPOP; ++p) pop_of[g.pops[ks.pops[p]]].push_back(p);
+        for (int u = 0; u < U; ++u) {
+            s << "    if (ok" << u << ") {\n";
+            s << "      const u64x2 m = {word_mask(w" << u << ", D.lw, D.lastmask), word_mask(w" << u << " + 1, D.lw, D.lastmask)};\n";
+            auto ref = [&](int32_t a) -> std::string {
+                const GNode& x = g.nodes[a];
+                if (in_kernel.count(a)) return "v" + std::to_string(a);
+                if (x.op == NOp::Zero) return "zero2";
+                if (x.op == NOp::Ones) return "m";
+                return "x" + std::to_string(a) + "_" + std::to_string(u);
+            };
+            auto sink = [&](int32_t n) {
+                auto oi = out_of.find(n);
+                auto pi = pop_of.find(n);
+                if (oi == out_of.end() && pi == pop_of.end()) return;
+                s << "      {\n        const u64x2 t = " << ref(n) << " & m;\n";
+                if (oi != out_of.end())
+                    for (int j : oi->second)
+                        s << "        if (lv" << u << ") lane_st16(urow_w(out" << j << " + ru" << u << " * 64ull), off16, t);\n";
+                if (pi != pop_of.end())
+                    for (int p : pi->second)
+                        s << "        pc" << p << " += (u32)(__builtin_popcountll(t.x) + __builtin_popcountll(t.y));\n";
+                s << "      }\n";
+                if (opt.store_order) s << "      __builtin_amdgcn_sched_barrier(0);\n";
+            };
+            for (int32_t n : ks.nodes) {
+                const GNode& x = g.nodes[n];
+                const std::string v = "v" + std::to_string(n);
+                switch (x.op) {
+                    case NOp::Zero: s << "      const u64x2 " << v << " = zero2;\n"; break;
+                    case NOp::Ones: s << "      const u64x2 " << v << " = m;\n"; break;
+                    case NOp::Not: s << "      const u64x2 " << v << " = ~" << ref(x.a) << ";\n"; break;
+                    case NOp::And: s << "      const u64x2 " << v << " = " << ref(x.a) << " & " << ref(x.b) << ";\n"; break;
+                    case NOp::Or: s << "      const u64x2 " << v << " = " << ref(x.a) << " | " << ref(x.b) << ";\n"; break;
+                    case NOp::Xor: s << "      const u64x2 " << v << " = " << ref(x.a) << " ^ " << ref(x.b) << ";\n"; break;
+                    case NOp::Shl: case NOp::Shr:  // only shifts by 0 reach a wide kernel
+                        s << "      const u64x2 " << v << " = " << ref(x.a) << ";\n";
+                        break;
+                    default: throw std::logic_error("wide kernel with a cross-lane op");
+                }
+                sink(n);
+            }
+            std::set<int32_t> done(ks.nodes.begin(), ks.nodes.end()), rest;
+            for (auto& kv : out_of) if (!done.count(kv.first)) rest.insert(kv.first);
+            for (auto& kv : pop_of) if (!done.count(kv.first)) rest.insert(kv.first);
+            for (int32_t n : rest) sink(n);
+            s << "    }\n";
+        }
+        s << "  }\n";
+        for (int p = 0; p < NPOP; ++p) {
+            s << "  {\n    const u32 tot = uni(wave_sum(pc" << p << "));\n";
+            s << "    if (lane == 0) __hip_atomic_fetch_add(&bank[(((u64)ii * " << NPOP << "u + " << p
+              << "u) << A.spread_log) + (c & ((1u << A.spread_log) - 1u))], (u64)tot, __ATOMIC_RELAXED, AGENT);\n  }\n";
+        }
+        s << "}\n";
+        ks.source = s.str();
+        return;
+    }
     const std::string NS = std::to_string(n_state) + "u";
     s << "extern \"C\" __global__ __launch_bounds__(" << LB << ") __attribute__((amdgpu_waves_per_eu(" << occ
       << "))) void bsz_kernel(KArgs A) {\n";
@@ -429,6 +580,9 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
             if (pi != pop_of.end())
                 for (int p : pi->second) s << "          pc" << p << " += (u32)__builtin_popcountll(" << ref(n) << " & m);\n";
             if (guard) s << "        }\n";
+            // without it the scheduler may sink every store to the end of the
+            // step and keep all the outputs of a row live at once
+            if (opt.store_order) s << "        __builtin_amdgcn_sched_barrier(0);\n";
         };
         for (int32_t n : ks.nodes) {
             const GNode& x = g.nodes[n];
@@ -545,7 +699,7 @@ std::string SegGraph::key() const {
 std::string PlanOptions::key() const {
     std::ostringstream s;
     s << chunk_rows << "," << warmup_rows << "," << waves_per_block << "," << rows_per_step << "," << max_ops << ","
-      << max_live << "," << occupancy;
+      << max_live << "," << occupancy << "," << words_per_lane << "," << store_order << "," << nontemporal;
     return s.str();
 }
 
@@ -735,7 +889,9 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
     for (KernelSpec& ks : P.kernels) {
         gen_kernel(g, opt, ks, emit);
         if (!emit) continue;
-        ks.unit = std::string(kKernelPrelude) + ks.source;
+        ks.unit = (opt.nontemporal > 0 ? "#define BSZ_NT " + std::to_string(std::min(2, opt.nontemporal)) + "\n"
+                                        : std::string()) +
+                  std::string(kKernelPrelude) + ks.source;
         mod << "// ---- " << ks.name << " ----\n" << ks.source;
     }
     P.module_source = mod.str();

@@ -21,6 +21,7 @@ others and the model only chooses which one to build.
 from __future__ import annotations
 
 import itertools
+import json
 import math
 import os
 import time
@@ -93,10 +94,17 @@ def measure(ctx: api.Context, source: str, case, candidates: list[dict], steps: 
     finally:
         res.free()
     words = max(1, (case.bitlength + 63) // 64)
-    F = costmodel.features_batch(prog, shape, [t.options for t in tr.trials], words, 1)
-    for t, f in zip(tr.trials, F):
-        out.options.append(t.options)
-        out.ms.append(t.ms)
+    # a candidate that partitions exactly like an earlier one runs its code
+    timed = [(t.options, t.ms) for t in tr.trials]
+    ms_of = {json.dumps(o, sort_keys=True): ms for o, ms in timed}
+    for cand, first in tr.same_as:
+        k = json.dumps(first, sort_keys=True)
+        if k in ms_of:
+            timed.append((cand, ms_of[k]))
+    F = costmodel.features_batch(prog, shape, [o for o, _ in timed], words, 1)
+    for (o, ms), f in zip(timed, F):
+        out.options.append(o)
+        out.ms.append(ms)
         out.features.append(f)
     out.measure_s = time.perf_counter() - t1
     return out
@@ -121,6 +129,17 @@ def collect(ctx: api.Context, specs: list[ProgramSpec], per_program: int = 16, s
             log(f"collect {i + 1}/{len(specs)}: {sp.nodes} nodes, {sp.n_bytes} bytes, best {min(m.ms):.3f} ms, "
                 f"worst {max(m.ms):.3f} ms (compile {m.compile_s:.1f} s, measure {m.measure_s:.1f} s)")
     return np.array(X, dtype=np.float32), np.array(y, dtype=np.float64), np.array(gid)
+
+
+def first_clean(ctx: api.Context, source: str, case, candidates: list[dict], pred: np.ndarray,
+                processes: int = 16) -> "Measured":
+    """Compile and time candidates in predicted order until one compiles
+    within its register file (a schedule the autotuner would accept)."""
+    for k in np.argsort(pred):
+        m = measure(ctx, source, case, [candidates[int(k)]], processes=processes)
+        if m.ms:
+            return m
+    raise RuntimeError("no candidate schedule compiles within the register file")
 
 
 @dataclass
@@ -154,9 +173,8 @@ def evaluate(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSp
         shape = tuning.shape_case([case])
         F = costmodel.features_batch(prog, shape, candidates, max(1, (case.bitlength + 63) // 64), 1)
         pred = model.predict_ms(F)
-        k = int(np.argmin(pred))
         feat_s = time.perf_counter() - t1
-        pick = measure(ctx, p.source, case, [candidates[k]], processes=processes)
+        pick = first_clean(ctx, p.source, case, candidates, pred, processes)
         model_s = time.perf_counter() - t1
         # the autotuner: compile + check + time every candidate
         t0 = time.perf_counter()
@@ -165,8 +183,12 @@ def evaluate(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSp
         best = min(m.ms)
         # the schedule without any search, for the ablation
         d = measure(ctx, p.source, case, [default], processes=processes)
-        mape = float(np.mean(np.abs(pred - np.array(m.ms)) / np.array(m.ms)))
-        row = EvalRow(sp.nodes, sp.n_bytes, best, pick.ms[0], d.ms[0], tune_s, model_s, mape)
+        measured = {json.dumps(o, sort_keys=True): t for o, t in zip(m.options, m.ms)}
+        pairs = [(pr, measured[json.dumps(c, sort_keys=True)]) for pr, c in zip(pred, candidates)
+                 if json.dumps(c, sort_keys=True) in measured]
+        mape = float(np.mean([abs(a - b) / b for a, b in pairs]))
+        row = EvalRow(sp.nodes, sp.n_bytes, best, pick.ms[0], d.ms[0] if d.ms else float("nan"), tune_s, model_s,
+                      mape)
         rows.append(row)
         if log:
             log(f"eval {sp.nodes} nodes / {sp.n_bytes} B: best {best:.3f} ms, pick {row.pick_ms:.3f} ms "
@@ -207,12 +229,14 @@ def mixed_batch(ctx: api.Context, model: costmodel.CostModel, specs: list[Progra
     of the programs as judge."""
     built = [sp.build() for sp in specs]
     t0 = time.perf_counter()
-    picks = []
+    preds = []
     for (p, case) in built:
         prog = api.Program(p.source)
         F = costmodel.features_batch(prog, tuning.shape_case([case]), candidates, max(1, (case.bitlength + 63) // 64), 1)
-        picks.append(candidates[int(np.argmin(model.predict_ms(F)))])
+        preds.append(model.predict_ms(F))
     pick_s = time.perf_counter() - t0
+    picks = [first_clean(ctx, p.source, case, candidates, pr, processes).options[0]
+             for (p, case), pr in zip(built, preds)]
     t1 = time.perf_counter()
     for (p, case), opt in zip(built, picks):
         tuning.precompile(p.source, [case], [opt, {}], processes=processes)
@@ -253,7 +277,7 @@ def mixed_batch(ctx: api.Context, model: costmodel.CostModel, specs: list[Progra
     for i in sorted(rng.choice(len(specs), size=n_judge, replace=False).tolist()):
         p, case = built[i]
         m = measure(ctx, p.source, case, candidates, processes=processes)
-        k = candidates.index(picks[i])
+        k = m.options.index(picks[i])
         res.judged.append((i, float(m.ms[k]), float(min(m.ms))))
         if log:
             log(f"judge program {i} ({specs[i].nodes} nodes, {specs[i].n_bytes} B): model pick {m.ms[k]:.3f} ms, "

@@ -1,13 +1,15 @@
 """Measured autotuner for fused plans.
 
 A schedule is a set of FusedOptions keys (chunk_rows, rows_per_step,
-waves_per_block, occupancy, warmup_rows, max_ops, max_live): the rows a wave
-walks per chunk, the rows whose loads are issued together (words per lane in
-flight), the workgroup width, the register budget and the fusion depth.  The
-tuner builds one plan per candidate over the same resident inputs, checks
-that every candidate produces the same popcounts and the same output words as
-the first one, and times each with hipEvents over replays of a captured
-hipGraph.  Schedules never change output bits; the check guards that.
+waves_per_block, occupancy, words_per_lane, warmup_rows, max_ops, max_live):
+the rows a wave walks per chunk, the rows whose loads are issued together
+(words per lane in flight), the workgroup width, the register budget, 8- or
+16-byte lanes and the fusion depth.  The tuner builds one plan per candidate
+over the same resident inputs, skips any whose compiled kernels use AGPRs or
+scratch memory (read from the code objects' metadata), checks that every
+other candidate produces the same popcounts and the same output words as the
+first one, and times each with hipEvents over replays of a captured hipGraph.
+Schedules never change output bits; the check guards that.
 
 The reference has no schedule search (one kernel per op, fixed 256-thread
 blocks: kernels.cuh:7-36, cuda_backend.cuh:53-84).
@@ -16,6 +18,7 @@ blocks: kernels.cuh:7-36, cuda_backend.cuh:53-84).
 from __future__ import annotations
 
 import itertools
+import json
 import time
 from dataclasses import dataclass, field
 
@@ -172,8 +175,8 @@ def default_space(kind: str = "auto") -> list[dict]:
         "waves_per_block": [4],
     }
     if kind == "wide":
-        grid = {"chunk_rows": [16, 32], "rows_per_step": [1, 2], "occupancy": [1, 2, 4],
-                "waves_per_block": [2, 4]}
+        grid = {"chunk_rows": [32, 64], "rows_per_step": [1, 2], "occupancy": [1, 2, 4],
+                "waves_per_block": [4], "words_per_lane": [1, 2], "max_live": [36, 96]}
     elif kind == "stateful":
         grid = {"chunk_rows": [4, 8, 16, 32], "rows_per_step": [2, 4, 8, 16], "occupancy": [0],
                 "waves_per_block": [1, 4]}
@@ -192,9 +195,21 @@ class Trial:
     compile_s: float
 
 
+def register_clean(info: dict) -> bool:
+    """Every compiled kernel of the plan stays in its architectural VGPRs
+    (no AGPR use, no scratch); kernels that overflowed have given wrong
+    results, so the tuner does not use such schedules."""
+    for k in info.get("kernels", []):
+        if k.get("agprs", 0) > 0 or k.get("scratch_bytes", 0) > 0:
+            return False
+    return True
+
+
 @dataclass
 class TuneResult:
     trials: list[Trial] = field(default_factory=list)
+    skipped: list = field(default_factory=list)
+    same_as: list = field(default_factory=list)   # (candidate, earlier candidate with the same kernels)
     seconds: float = 0.0
 
     @property
@@ -203,9 +218,25 @@ class TuneResult:
 
     def as_json(self) -> dict:
         b = self.best
-        return {"candidates": len(self.trials), "tuning_s": round(self.seconds, 2),
+        return {"candidates": len(self.trials), "skipped": len(self.skipped), "duplicates": len(self.same_as),
+                "tuning_s": round(self.seconds, 2),
                 "best": {"options": b.options, "ms": round(b.ms, 5)},
                 "worst_ms": round(max(t.ms for t in self.trials), 5)}
+
+
+_LAUNCH_KEYS = ("waves_per_block", "warmup_rows", "store_order", "nontemporal")
+
+
+def plan_keys(program: api.Program, shape: api.Case, options_list: list[dict]) -> list[str]:
+    """A key per schedule that is equal exactly when two schedules partition
+    the program into the same kernels with the same shapes and launch them
+    the same way (so they compile to the same code)."""
+    descs = api.describe_plans(program, shape, [dict(o, detail=1) for o in options_list])
+    out = []
+    for o, d in zip(options_list, descs):
+        launch = {k: o.get(k) for k in _LAUNCH_KEYS}
+        out.append(json.dumps([d["kernels"], d.get("kernel_detail"), d["chunk_rows"], launch], sort_keys=True))
+    return out
 
 
 def autotune(ctx: api.Context, program: api.Program, residents: list[Resident], space: list[dict],
@@ -219,7 +250,13 @@ def autotune(ctx: api.Context, program: api.Program, residents: list[Resident], 
     res = TuneResult()
     shape = shape_case(residents[0].cases)
     ref_sig = None
-    for cand in space:
+    keys = plan_keys(program, shape, [dict(base_options or {}, **c) for c in space])
+    first_of = {}
+    for cand, key in zip(space, keys):
+        if key in first_of:
+            res.same_as.append((cand, first_of[key]))
+            continue
+        first_of[key] = cand
         opts = dict(base_options or {})
         opts.update(cand)
         tc = time.perf_counter()
@@ -228,6 +265,13 @@ def autotune(ctx: api.Context, program: api.Program, residents: list[Resident], 
         try:
             for p, r in zip(plans, residents):
                 r.bind(p)
+            info = plans[0].info()
+            if not register_clean(info):
+                res.skipped.append((cand, "kernels use AGPRs or scratch"))
+                if log:
+                    log(f"tune {cand}: skipped (compiled kernels use AGPRs or scratch)")
+                continue
+            for p in plans:
                 p.run()
             ctx.sync()
             sig = tuple(plan_signature(p, r, int_names, out_names) for p, r in zip(plans, residents))
@@ -236,11 +280,13 @@ def autotune(ctx: api.Context, program: api.Program, residents: list[Resident], 
             elif sig != ref_sig:
                 raise RuntimeError(f"schedule {cand} changed the outputs")
             ms = time_runs(ctx, plans, steps, warmup)
-            res.trials.append(Trial(cand, ms, plans[0].info(), compile_s))
+            res.trials.append(Trial(cand, ms, info, compile_s))
         finally:
             for p in plans:
                 p.close()
         if log:
             log(f"tune {cand}: {ms:.4f} ms")
     res.seconds = time.perf_counter() - t0
+    if not res.trials:
+        raise RuntimeError(f"all {len(space)} candidate schedules compile to kernels that use AGPRs or scratch")
     return res

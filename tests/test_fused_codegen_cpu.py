@@ -97,3 +97,47 @@ def test_describe_matches_compile_only_shapes():
         info = api.compile_plan_only(prog, shape, dict(o, compile=0))
         keys = ("nodes", "inputs", "outputs", "pops", "state_words", "rows_per_step", "occupancy")
         assert [[k[x] for x in keys] for k in d["kernels"]] == [[k[x] for x in keys] for k in info["kernels"]]
+
+
+@pytest.mark.parametrize("occ", [1, 2])
+def test_adder_kernel_fits_its_register_file(occ):
+    """The 32-bit adder fuses into one kernel with 65 input and 33 output
+    streams.  Its compiled code object (metadata read on the host) stays in
+    the architectural VGPRs of the requested occupancy: no AGPRs, no scratch.
+    Without the store ordering the compiler sinks all 33 stores to the end of
+    the row and spills."""
+    w = workloads.adder32(1 << 16, seed=1)
+    info = api.compile_plan_only(api.Program(w.source), api.Case.from_bscase(w.cases[0]),
+                                 {"chunk_rows": 32, "rows_per_step": 1, "occupancy": occ, "waves_per_block": 4})
+    (k,) = info["kernels"]
+    assert k["scratch_bytes"] == 0 and k["agprs"] == 0, k
+    assert 0 < k["vgprs"] <= 512 // occ and k["vgprs"] <= 256, k
+
+
+def test_two_words_per_lane_shapes():
+    """Kernels without carries or advances take 16-byte lanes; stateful ones
+    in the same plan stay at 8 bytes."""
+    src = """input stream a
+input stream b
+output stream x
+output stream y
+x = a ^ b
+stream s = a + b
+y = s ^ x
+"""
+    case = BsCase(bitlength=64 * 64 * 4, inputs={"a": np.zeros(256, np.uint64), "b": np.zeros(256, np.uint64)})
+    opts = {"chunk_rows": 8, "rows_per_step": 2, "words_per_lane": 2, "max_ops": 1}
+    info = api.compile_plan_only(api.Program(src), api.Case.from_bscase(case), opts)
+    wpl = {k["state_words"] > 0: k["words_per_lane"] for k in info["kernels"]}
+    assert wpl == {False: 2, True: 1}, info["kernels"]
+
+
+def test_plan_keys_merge_equivalent_schedules():
+    """The autotuner times a partition once: schedules whose kernels and
+    launches coincide share a key, others do not."""
+    from bitstreamzoo_amd import tuning
+    w = workloads.matchstar(64 * 64 * 4, seed=1)
+    space = [{"chunk_rows": 4, "rows_per_step": 4}, {"chunk_rows": 4, "rows_per_step": 8},
+             {"chunk_rows": 8, "rows_per_step": 8}, {"chunk_rows": 4, "rows_per_step": 4, "waves_per_block": 1}]
+    k = tuning.plan_keys(api.Program(w.source), tuning.shape_case(w.cases), space)
+    assert k[0] == k[1] and len({k[0], k[2], k[3]}) == 3

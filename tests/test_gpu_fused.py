@@ -165,3 +165,51 @@ def test_random_straight_line_plans(seed):
     ora, _ = run_model(Oracle, src, case, Oracle.SCALAR, stream_names=names[0], int_names=names[1])
     got, _ = run_plan(src, [case], TILES[seed % len(TILES)], names)
     assert_same(got[0], ora, case.bitlength, set(names[1]), f"seed {seed}")
+
+
+WIDE_TILES = [{"chunk_rows": 2, "rows_per_step": 1, "words_per_lane": 2},
+              {"chunk_rows": 16, "rows_per_step": 2, "words_per_lane": 2},
+              {"chunk_rows": 32, "rows_per_step": 8, "words_per_lane": 2, "occupancy": 2}]
+
+
+@pytest.mark.parametrize("tile", WIDE_TILES, ids=[str(t) for t in WIDE_TILES])
+@pytest.mark.parametrize("bl", [1, 64 * 63 + 5, 64 * 64 * 7 + 64 * 3, 64 * 64 * 33 + 1])
+def test_two_words_per_lane_adder(tile, bl):
+    """16-byte lanes over an odd number of rows, a last row shorter than a
+    lane pair, and chunks that end mid-stream."""
+    w = workloads.adder32(bl, seed=bl + 1, width=8)
+    ora, names = oracle_outs(w.source, w.cases[0])
+    got, _ = run_plan(w.source, w.cases, tile, names)
+    assert_same(got[0], ora, bl, set(names[1]), f"adder8 {tile}")
+
+
+def test_two_words_per_lane_is_used_and_batches():
+    """The adder's kernels take two words per lane; a batch of uneven
+    instances (each its own chunk range) matches the oracle per instance."""
+    cases = [workloads.adder32(n, seed=n, width=4).cases[0] for n in (100, 64 * 64 * 2 + 1, 64 * 70, 5)]
+    src = workloads.adder32_source(4)
+    plan = api.Plan(ctx(), api.Program(src), api.Case.from_bscase(cases[0]), WIDE_TILES[1])
+    assert all(k["words_per_lane"] == 2 for k in plan.info()["kernels"])
+    plan.close()
+    names = Oracle(src).output_names()
+    got, _ = run_plan(src, cases, WIDE_TILES[1], names)
+    for i, case in enumerate(cases):
+        ora, _ = run_model(Oracle, src, case, Oracle.SCALAR, stream_names=names[0], int_names=names[1])
+        assert_same(got[i], ora, case.bitlength, set(names[1]), f"instance {i}")
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_straight_line_plans_two_words_per_lane(seed):
+    """Random programs (popcounts included); kernels with carries or
+    advances fall back to one word per lane within the same plan."""
+    src = progfuzz.random_program(100 + seed, n_stmts=40, allow_control=False, host_ints=False)
+    case = progfuzz.random_case(100 + seed, [777, 4096 * 2 + 5, 64 * 64 * 9 + 1][seed % 3])
+    case.params["P"] = case.params["P"] % 64
+    names = Oracle(src).output_names()
+    try:
+        api.compile_plan_only(api.Program(src), api.Case.from_bscase(case))
+    except Exception:
+        pytest.skip("program reads a popcount on the host")
+    ora, _ = run_model(Oracle, src, case, Oracle.SCALAR, stream_names=names[0], int_names=names[1])
+    got, _ = run_plan(src, [case], WIDE_TILES[seed % len(WIDE_TILES)], names)
+    assert_same(got[0], ora, case.bitlength, set(names[1]), f"seed {seed}")

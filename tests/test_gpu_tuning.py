@@ -18,7 +18,11 @@ def test_autotune_adder_small():
              {"chunk_rows": 8, "rows_per_step": 2, "occupancy": 1},
              {"chunk_rows": 16, "rows_per_step": 1, "occupancy": 4, "waves_per_block": 2}]
     tr = tuning.autotune(ctx, api.Program(w.source), [res], space, ["n_" + o for o in outs], outs, steps=3, warmup=1)
-    assert len(tr.trials) == 3
+    # two rows of 65 inputs in flight at occupancy 1 overflow into AGPRs
+    # (the code object's metadata says so): that schedule is not timed
+    assert [t.options for t in tr.trials] == [space[0], space[2]]
+    assert [c for c, _ in tr.skipped] == [space[1]]
+    assert all(k["agprs"] == 0 and k["scratch_bytes"] == 0 for t in tr.trials for k in t.info["kernels"])
     assert tr.best.ms == min(t.ms for t in tr.trials)
     assert all(t.ms > 0 for t in tr.trials)
     res.free()
@@ -30,7 +34,9 @@ def test_autotune_matchstar_rotations():
     residents = [tuning.Resident(ctx, w.cases) for w in ws]
     space = tuning.default_space("stateful")[:6]
     tr = tuning.autotune(ctx, api.Program(ws[0].source), residents, space, ["n"], ["result"], steps=2, warmup=1)
-    assert len(tr.trials) == 6
+    # rows per step divide the chunk: 8 rows per step over 4-row chunks is
+    # the 4-row schedule again, and is timed once
+    assert len(tr.trials) + len(tr.same_as) == 6 and len(tr.trials) == 4
     # the chosen schedule reproduces the independent big-int result
     plan = api.Plan(ctx, api.Program(ws[1].source), tuning.shape_case(ws[1].cases), tr.best.options)
     residents[1].bind(plan)
