@@ -256,28 +256,34 @@ def _get_str(fn, h) -> str:
     return buf.value.decode()
 
 
+def _fill_str(call, cap: int = 1 << 16) -> str:
+    """The string of a `call(buf, cap) -> full length` entry point, from one
+    call whose whole output fitted.  (Two calls, one for the size and one for
+    the text, can disagree: a compile-only plan reports the register use of
+    code objects that other processes may finish in between.)"""
+    for _ in range(8):
+        buf = C.create_string_buffer(cap)
+        n = int(call(buf, cap))
+        if n < 0:
+            check(n)
+        if n < cap:
+            return buf.raw[:n].decode()
+        cap = n + n // 4 + 1
+    raise RuntimeError("output kept growing between calls")
+
+
 def compile_plan_only(program: "Program", shape: "Case", options: dict | None = None) -> dict:
     """Record + generate + hipRTC-compile a fused plan (no GPU needed)."""
     opts = json.dumps(options or {}).encode()
     L = lib()
-    n = L.bsz_plan_compile_only(program.h, shape.h, opts, None, 0)
-    if n < 0:
-        check(int(n))
-    buf = C.create_string_buffer(n + 1)
-    L.bsz_plan_compile_only(program.h, shape.h, opts, buf, n + 1)
-    return json.loads(buf.value.decode())
+    return json.loads(_fill_str(lambda buf, cap: L.bsz_plan_compile_only(program.h, shape.h, opts, buf, cap)))
 
 
 def describe_plans(program: "Program", shape: "Case", options_list: list[dict]) -> list[dict]:
     """Kernel shapes of the program under each schedule (no code, no device)."""
     js = json.dumps(options_list).encode()
     L = lib()
-    n = L.bsz_plan_describe(program.h, shape.h, js, None, 0)
-    if n < 0:
-        check(int(n))
-    buf = C.create_string_buffer(n + 1)
-    L.bsz_plan_describe(program.h, shape.h, js, buf, n + 1)
-    return json.loads(buf.value.decode())
+    return json.loads(_fill_str(lambda buf, cap: L.bsz_plan_describe(program.h, shape.h, js, buf, cap)))
 
 
 class Plan:

@@ -141,3 +141,19 @@ def test_plan_keys_merge_equivalent_schedules():
              {"chunk_rows": 8, "rows_per_step": 8}, {"chunk_rows": 4, "rows_per_step": 4, "waves_per_block": 1}]
     k = tuning.plan_keys(api.Program(w.source), tuning.shape_case(w.cases), space)
     assert k[0] == k[1] and len({k[0], k[2], k[3]}) == 3
+
+
+def test_string_results_come_from_one_complete_call():
+    """A compile-only plan's info can grow between calls (kernels compiled by
+    other processes meanwhile report their registers); the text returned is
+    always one call's whole output."""
+    import ctypes as C
+    from bitstreamzoo_amd.api import _fill_str
+    outs = iter(['{"a":1}', '{"a":"' + "x" * 70000 + '"}', '{"a":"' + "x" * 70000 + '"}'])
+
+    def call(buf, cap):
+        t = next(outs).encode()
+        C.memmove(buf, t, min(len(t), cap - 1))
+        return len(t)
+
+    assert _fill_str(call, cap=4) == '{"a":"' + "x" * 70000 + '"}'
