@@ -233,6 +233,59 @@ def bench_shift_and(ctx, args):
                "plan": info, "instances": n, "checked_instances": len(check_idx), "options": opts, "tune": tune}
 
 
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9   # CUs x 64 lanes x 2.4 GHz: 32-bit VALU lane-ops per second
+
+
+def bench_regex_dag(ctx, args):
+    """A generated regex-style DAG (default 200k generator nodes, ~250k DSL
+    ops) over the 8 basis streams of a 2^27-byte Zipf text.  Untuned default
+    schedule (its kernels are compiled by 16 worker processes first).  All 64
+    outputs of the fused plan are compared word for word with the per-op
+    executor's on the same inputs."""
+    from bitstreamzoo_amd import dag_gen
+    nodes = args.dag_nodes
+    nbytes = args.bitlength or (1 << 27)
+    log(f"regex_dag: generating {nodes} nodes over a {nbytes}-byte text")
+    p = dag_gen.generate(nodes, seed=4)
+    case = dag_gen.text_case(nbytes, seed=11)
+    prog = api.Program(p.source)
+    opts = json.loads(args.schedule) if args.schedule else {}
+    t0 = time.perf_counter()
+    tuning.precompile(p.source, [case], [opts], processes=16, shards=64, log=lambda m: log(f"regex_dag: {m}"))
+    compile_s = time.perf_counter() - t0
+    res = tuning.Resident(ctx, [case])
+    plan = api.Plan(ctx, prog, tuning.shape_case([case]), opts)
+    res.bind(plan)
+    info = plan.info()
+    ks = info["kernels"]
+    log(f"regex_dag: {len(ks)} kernels compiled in {compile_s:.1f} s; running the fused plan")
+    nw = (case.bitlength + 63) // 64
+    plan.run()
+    ctx.sync()
+    fused = {o: plan.download_output(0, o, nw) for o in p.outputs}
+    log("regex_dag: running the per-op executor for the output check")
+    plan.set_executor(api.PEROP)
+    plan.run()
+    ctx.sync()
+    bad = [o for o in p.outputs if not np.array_equal(plan.download_output(0, o, nw), fused[o])]
+    if bad:
+        raise SystemExit(f"regex_dag: {len(bad)} fused outputs differ from the per-op executor's")
+    del fused
+    launches = len(ks) + sum(1 for k in ks if k["state_words"] > 0) + (0 if ks[-1]["state_words"] else 1)
+    ms = time_plans(ctx, [plan], args.steps, args.warmup, api.FUSED, launches_per_run=launches)
+    log(f"regex_dag: fused {ms:.1f} ms/step")
+    perop = time_plans(ctx, [plan], 1, 0, api.PEROP, launches_per_run=info["graph_nodes"])
+    log(f"regex_dag: per-op {perop:.1f} ms/step")
+    plan.close()
+    res.free()
+    planned = 8 * nw * sum(k["inputs"] + k["outputs"] for k in ks)
+    alg = 8 * nw * (8 + len(p.outputs))
+    w = workloads.Workload("regex_dag", p.source, [case], bit_ops=p.n_dsl_ops * case.bitlength, alg_bytes=alg)
+    return w, {"ms_per_step": ms, "perop_ms_per_step": perop, "bit_ops": w.bit_ops, "alg_bytes": alg,
+               "planned_bytes": planned, "lane_ops": 2 * p.n_dsl_ops * nw, "plan": info, "options": opts,
+               "compile_s": compile_s, "dsl_ops": p.n_dsl_ops, "outputs_checked": len(p.outputs)}
+
+
 def bench_learned(ctx, args):
     """The learned cost model against the measured autotuner on held-out
     generated DAG programs: train the GPU MLP on the committed autotuner
@@ -297,7 +350,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="adder32", choices=["adder32", "matchstar", "shift_and"])
+    ap.add_argument("--workload", default="adder32", choices=["adder32", "matchstar", "shift_and", "regex_dag"])
+    ap.add_argument("--dag-nodes", type=int, default=200_000, help="generator nodes of the regex_dag workload")
     ap.add_argument("--bitlength", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perop", action="store_true")
@@ -319,6 +373,8 @@ def main():
         w, r = bench_adder(ctx, args, perop=not args.no_perop)
     elif args.workload == "matchstar":
         w, r = bench_matchstar(ctx, args)
+    elif args.workload == "regex_dag":
+        w, r = bench_regex_dag(ctx, args)
     else:
         w, r = bench_shift_and(ctx, args)
     results[args.workload] = r
@@ -360,7 +416,10 @@ def main():
         "config": {"workload": {
             "adder32": "bitsliced 32-bit ripple-carry adder DAG, 65 inputs x 2^28 bits, 160 ops + 33 popcounts",
             "matchstar": "MatchStar on 2 x 2^26-bit streams + popcount, 12 rotating input sets",
-            "shift_and": "batched Shift-And, 2048 sequences, 32-mer, one launch"}[args.workload],
+            "shift_and": "batched Shift-And, 2048 sequences, 32-mer, one launch",
+            "regex_dag": f"random regex-style DAG, {args.dag_nodes} generator nodes "
+                         f"({r.get('dsl_ops')} DSL ops), 8 basis streams of a "
+                         f"{w.cases[0].bitlength}-byte Zipf(1.1) text, 64 outputs"}[args.workload],
             "executor": "fused", "bitlength": w.cases[0].bitlength if args.workload != "shift_and" else None,
             "kernels": len(r["plan"]["kernels"]), "schedule": r["options"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -368,6 +427,16 @@ def main():
                      "traffic": profiled_traffic() if args.workload == "adder32" else None,
                      "measured_copy_GBps": round(copy_gbps, 1)},
     }
+    if args.workload == "regex_dag":
+        # HBM bytes the plan moves (intermediates between kernels included)
+        # and the VALU bound the workload is quoted against
+        line["roofline"]["planned_bytes"] = r["planned_bytes"]
+        line["roofline"]["planned_frac"] = round(r["planned_bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+        line["valu"] = {"lane_ops": r["lane_ops"], "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+                        "bound_ms": round(r["lane_ops"] / VALU_PEAK_LANE_OPS * 1e3, 2),
+                        "frac": round(r["lane_ops"] / VALU_PEAK_LANE_OPS / (ms * 1e-3), 4)}
+        line["compile_s"] = round(r["compile_s"], 1)
+        line["outputs_checked_against_perop"] = r["outputs_checked"]
     if r.get("tune"):
         line["autotune"] = r["tune"]
     if "perop_ms_per_step" in r:

@@ -276,14 +276,16 @@ def compile_plan_only(program: "Program", shape: "Case", options: dict | None = 
     """Record + generate + hipRTC-compile a fused plan (no GPU needed)."""
     opts = json.dumps(options or {}).encode()
     L = lib()
-    return json.loads(_fill_str(lambda buf, cap: L.bsz_plan_compile_only(program.h, shape.h, opts, buf, cap)))
+    return json.loads(_fill_str(lambda buf, cap: L.bsz_plan_compile_only(program.h, shape.h, opts, buf, cap),
+                                cap=1 << 22))
 
 
 def describe_plans(program: "Program", shape: "Case", options_list: list[dict]) -> list[dict]:
     """Kernel shapes of the program under each schedule (no code, no device)."""
     js = json.dumps(options_list).encode()
     L = lib()
-    return json.loads(_fill_str(lambda buf, cap: L.bsz_plan_describe(program.h, shape.h, js, buf, cap)))
+    return json.loads(_fill_str(lambda buf, cap: L.bsz_plan_describe(program.h, shape.h, js, buf, cap),
+                                cap=1 << 22))
 
 
 class Plan:

@@ -46,21 +46,28 @@ def _compile_worker(job):
     return True
 
 
-def precompile(source: str, cases: list[BsCase], options_list: list[dict], processes: int = 16) -> float:
+def precompile(source: str, cases: list[BsCase], options_list: list[dict], processes: int = 16,
+               shards: int = 0, log=None) -> float:
     """Fill the code-object cache for every schedule of `options_list` with
     `processes` worker processes (hipRTC compiles one unit at a time per
-    process); each plan's kernels are sharded over the workers.  Returns the
-    wall seconds spent.  Needs no GPU."""
+    process); each plan's kernels are split into `shards` (default
+    `processes`) jobs over the workers, and `log` hears of every finished job.
+    Returns the wall seconds spent.  Needs no GPU."""
+    import concurrent.futures as cf
     t0 = time.perf_counter()
     desc = _shape_desc(cases)
+    shards = shards or processes
     jobs = []
     for opts in options_list:
-        for r in range(processes):
+        for r in range(shards):
             o = dict(opts)
-            o.update(compile_shard=r, compile_shards=processes)
+            o.update(compile_shard=r, compile_shards=shards)
             jobs.append((source, desc, o))
-    for _ in _pool(processes).map(_compile_worker, jobs):
-        pass
+    futs = [_pool(processes).submit(_compile_worker, j) for j in jobs]
+    for i, f in enumerate(cf.as_completed(futs)):
+        f.result()
+        if log:
+            log(f"compiled {i + 1}/{len(jobs)} shards ({time.perf_counter() - t0:.0f} s)")
     return time.perf_counter() - t0
 
 

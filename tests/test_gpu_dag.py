@@ -116,3 +116,24 @@ def test_wide_schedules_match_oracle_without_spills(args, idx, sched):
     res.free()
     assert all(k.get("scratch_bytes", 0) == 0 for k in ks), [k.get("scratch_bytes") for k in ks]
     assert not bad, f"{len(bad)} outputs differ"
+
+
+def test_large_dag_matches_oracle():
+    """A 20k-node generated DAG (about 25k DSL ops, hundreds of kernels and
+    slots) at the default schedule."""
+    p = dag_gen.generate(20_000, seed=21)
+    case = dag_gen.text_case((1 << 18) + 777, seed=22)
+    exp = oracle_outputs(p, case)
+    tuning.precompile(p.source, [case], [{}], processes=8)
+    c = ctx()
+    res = tuning.Resident(c, [case])
+    plan = api.Plan(c, api.Program(p.source), tuning.shape_case([case]), {})
+    res.bind(plan)
+    plan.run()
+    nw = (case.bitlength + 63) // 64
+    n_kernels = len(plan.info()["kernels"])
+    bad = [n for n in p.outputs if words_to_int(plan.download_output(0, n, nw)) != exp[n]]
+    plan.close()
+    res.free()
+    assert n_kernels > 100
+    assert not bad, f"{len(bad)} of {len(p.outputs)} outputs differ"
