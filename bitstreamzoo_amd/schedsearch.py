@@ -81,7 +81,7 @@ def measure(ctx: api.Context, source: str, case, candidates: list[dict], steps: 
     """Compile every candidate (parallel processes), then check and time each."""
     out = Measured()
     t0 = time.perf_counter()
-    tuning.precompile(source, [case], candidates, processes=processes)
+    tuning.precompile(source, [case], candidates, processes=processes, log=log)
     out.compile_s = time.perf_counter() - t0
     t1 = time.perf_counter()
     prog = api.Program(source)
@@ -178,7 +178,7 @@ def evaluate(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSp
         model_s = time.perf_counter() - t1
         # the autotuner: compile + check + time every candidate
         t0 = time.perf_counter()
-        m = measure(ctx, p.source, case, candidates, processes=processes)
+        m = measure(ctx, p.source, case, candidates, processes=processes, log=log)
         tune_s = time.perf_counter() - t0
         best = min(m.ms)
         # the schedule without any search, for the ablation
@@ -276,7 +276,7 @@ def mixed_batch(ctx: api.Context, model: costmodel.CostModel, specs: list[Progra
     n_judge = max(1, int(round(judge_frac * len(specs))))
     for i in sorted(rng.choice(len(specs), size=n_judge, replace=False).tolist()):
         p, case = built[i]
-        m = measure(ctx, p.source, case, candidates, processes=processes)
+        m = measure(ctx, p.source, case, candidates, processes=processes, log=log)
         k = m.options.index(picks[i])
         res.judged.append((i, float(m.ms[k]), float(min(m.ms))))
         if log:

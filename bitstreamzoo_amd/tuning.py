@@ -64,9 +64,10 @@ def precompile(source: str, cases: list[BsCase], options_list: list[dict], proce
             o.update(compile_shard=r, compile_shards=shards)
             jobs.append((source, desc, o))
     futs = [_pool(processes).submit(_compile_worker, j) for j in jobs]
+    every = max(1, len(jobs) // 10)
     for i, f in enumerate(cf.as_completed(futs)):
         f.result()
-        if log:
+        if log and ((i + 1) % every == 0 or i + 1 == len(jobs)):
             log(f"compiled {i + 1}/{len(jobs)} shards ({time.perf_counter() - t0:.0f} s)")
     return time.perf_counter() - t0
 

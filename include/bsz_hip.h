@@ -132,14 +132,20 @@ int64_t bsz_result_json(const bsz_result* r, char* buf, uint64_t cap);
 /* Records the program with `shape`'s params (every input it reads must be
  * present in `shape`, any length) and compiles it.  `options_json` is "" or a
  * flat object with chunk_rows, warmup_rows, waves_per_block, rows_per_step,
- * max_ops, max_live.  BSZ_E_RUNTIME if the program needs host decisions
- * (if/while, int expressions that read a popcount). */
+ * max_ops, max_live, occupancy (0 = automatic), words_per_lane (1 or 2),
+ * store_order (1 = default) and nontemporal (0, 1 stores, 2 loads and
+ * stores).  BSZ_E_RUNTIME if the program needs host decisions (if/while, int
+ * expressions that read a popcount). */
 int bsz_plan_create(bsz_ctx* ctx, const bsz_program* p, const bsz_case* shape, const char* options_json,
                     bsz_plan** out);
 /* Record + generate + hipRTC-compile only (no device needed); writes the plan
- * description JSON.  Option "compile":0 stops before hipRTC (partition and
- * kernel shapes only); "compile_shard"/"compile_shards" compile a subset of the
- * kernels, so several processes can fill the code-object cache in parallel. */
+ * description JSON, with each compiled kernel's VGPR, AGPR and scratch use
+ * read from its code object.  Option "compile":0 stops before hipRTC
+ * (partition and kernel shapes only); "compile_shard"/"compile_shards" compile
+ * a subset of the kernels, so several processes can fill the code-object cache
+ * in parallel.  Returns the full length of the JSON; it may differ between
+ * calls while other processes fill the cache, so read it from a call whose
+ * `cap` exceeded the length. */
 int64_t bsz_plan_compile_only(const bsz_program* p, const bsz_case* shape, const char* options_json,
                               char* info, uint64_t cap);
 /* Kernel shapes (per kernel: nodes, inputs, outputs, popcounts, state words,
