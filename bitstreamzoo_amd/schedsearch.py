@@ -77,7 +77,7 @@ class Measured:
 
 
 def measure(ctx: api.Context, source: str, case, candidates: list[dict], steps: int = 5, processes: int = 16,
-            log=None) -> Measured:
+            log=None, allow_empty: bool = False) -> Measured:
     """Compile every candidate (parallel processes), then check and time each."""
     out = Measured()
     t0 = time.perf_counter()
@@ -90,7 +90,8 @@ def measure(ctx: api.Context, source: str, case, candidates: list[dict], steps: 
     names = [n for n in source.split("\n") if n.startswith("output stream ")]
     outs = [n.split()[-1] for n in names][:8]
     try:
-        tr = tuning.autotune(ctx, prog, [res], candidates, [], outs, steps=steps, warmup=1, log=log)
+        tr = tuning.autotune(ctx, prog, [res], candidates, [], outs, steps=steps, warmup=1, log=log,
+                             allow_empty=allow_empty)
     finally:
         res.free()
     words = max(1, (case.bitlength + 63) // 64)
@@ -101,7 +102,7 @@ def measure(ctx: api.Context, source: str, case, candidates: list[dict], steps: 
         k = json.dumps(first, sort_keys=True)
         if k in ms_of:
             timed.append((cand, ms_of[k]))
-    F = costmodel.features_batch(prog, shape, [o for o, _ in timed], words, 1)
+    F = costmodel.features_batch(prog, shape, [o for o, _ in timed], words, 1) if timed else []
     for (o, ms), f in zip(timed, F):
         out.options.append(o)
         out.ms.append(ms)
@@ -136,7 +137,7 @@ def first_clean(ctx: api.Context, source: str, case, candidates: list[dict], pre
     """Compile and time candidates in predicted order until one compiles
     within its register file (a schedule the autotuner would accept)."""
     for k in np.argsort(pred):
-        m = measure(ctx, source, case, [candidates[int(k)]], processes=processes)
+        m = measure(ctx, source, case, [candidates[int(k)]], processes=processes, allow_empty=True)
         if m.ms:
             return m
     raise RuntimeError("no candidate schedule compiles within the register file")
@@ -182,7 +183,7 @@ def evaluate(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSp
         tune_s = time.perf_counter() - t0
         best = min(m.ms)
         # the schedule without any search, for the ablation
-        d = measure(ctx, p.source, case, [default], processes=processes)
+        d = measure(ctx, p.source, case, [default], processes=processes, allow_empty=True)
         measured = {json.dumps(o, sort_keys=True): t for o, t in zip(m.options, m.ms)}
         pairs = [(pr, measured[json.dumps(c, sort_keys=True)]) for pr, c in zip(pred, candidates)
                  if json.dumps(c, sort_keys=True) in measured]
@@ -235,11 +236,16 @@ def mixed_batch(ctx: api.Context, model: costmodel.CostModel, specs: list[Progra
         F = costmodel.features_batch(prog, tuning.shape_case([case]), candidates, max(1, (case.bitlength + 63) // 64), 1)
         preds.append(model.predict_ms(F))
     pick_s = time.perf_counter() - t0
-    picks = [first_clean(ctx, p.source, case, candidates, pr, processes).options[0]
-             for (p, case), pr in zip(built, preds)]
+    picks = []
+    for i, ((p, case), pr) in enumerate(zip(built, preds)):
+        picks.append(first_clean(ctx, p.source, case, candidates, pr, processes).options[0])
+        if log:
+            log(f"mixed: program {i + 1}/{len(built)} pick {picks[-1]}")
     t1 = time.perf_counter()
-    for (p, case), opt in zip(built, picks):
+    for i, ((p, case), opt) in enumerate(zip(built, picks)):
         tuning.precompile(p.source, [case], [opt, {}], processes=processes)
+        if log:
+            log(f"mixed: compiled program {i + 1}/{len(built)}")
     compile_s = time.perf_counter() - t1
     residents, plans, defaults = [], [], []
     bit_ops = alg = 0
