@@ -46,6 +46,7 @@ FusedOptions FusedOptions::from_env() {
     o.words_per_lane = env_int("BSZ_WORDS_PER_LANE", o.words_per_lane);
     o.store_order = env_int("BSZ_STORE_ORDER", o.store_order);
     o.nontemporal = env_int("BSZ_NONTEMPORAL", o.nontemporal);
+    o.interleave = env_int("BSZ_INTERLEAVE", o.interleave);
     o.max_ops = env_int("BSZ_MAX_OPS", o.max_ops);
     o.max_live = env_int("BSZ_MAX_LIVE", o.max_live);
     o.occupancy = env_int("BSZ_OCCUPANCY", o.occupancy);
@@ -71,6 +72,7 @@ FusedOptions FusedOptions::from_json_obj(const Json& j) {
     get("words_per_lane", o.words_per_lane);
     get("store_order", o.store_order);
     get("nontemporal", o.nontemporal);
+    get("interleave", o.interleave);
     get("max_ops", o.max_ops);
     get("max_live", o.max_live);
     get("occupancy", o.occupancy);
@@ -90,6 +92,7 @@ PlanOptions to_plan_options(const FusedOptions& f) {
     p.words_per_lane = f.words_per_lane >= 2 ? 2 : 1;
     p.store_order = f.store_order != 0;
     p.nontemporal = std::max(0, std::min(2, f.nontemporal));
+    p.interleave = f.interleave != 0;
     p.max_ops = std::max(1, f.max_ops);
     p.max_live = std::max(4, f.max_live);
     p.occupancy = f.occupancy <= 0 ? 0 : std::min(8, f.occupancy);
@@ -1335,7 +1338,8 @@ std::string FusedPlan::info_json() const {
         o << (k ? "," : "") << "{\"name\":\"" << ks.name << "\",\"nodes\":" << ks.nodes.size()
           << ",\"inputs\":" << ks.inputs.size() << ",\"outputs\":" << ks.outputs.size() << ",\"pops\":" << ks.pops.size()
           << ",\"state_words\":" << ks.n_state << ",\"rows_per_step\":" << ks.rows_per_step
-          << ",\"occupancy\":" << ks.occupancy << ",\"words_per_lane\":" << ks.words_per_lane;
+          << ",\"occupancy\":" << ks.occupancy << ",\"words_per_lane\":" << ks.words_per_lane
+          << ",\"interleave\":" << (ks.interleave ? 1 : 0);
         if (I.mod && k < I.mod->scratch.size()) o << ",\"scratch_bytes\":" << I.mod->scratch[k];
         if (I.mod && k < I.mod->regs.size()) {
             o << ",\"vgprs\":" << I.mod->regs[k].vgprs << ",\"agprs\":" << I.mod->regs[k].agprs;

@@ -18,6 +18,7 @@ struct FusedOptions {
     int words_per_lane = 1;     // 2: 16-byte loads/stores in kernels without cross-lane work
     int store_order = 1;        // 1: each output's store stays where its value is computed
     int nontemporal = 0;        // 1: nontemporal stores of stream words, 2: loads too
+    int interleave = 0;         // 1: kernels without state take every n-th row instead of a contiguous chunk
     int max_ops = 4000;         // fusion depth: DAG nodes per kernel
     int max_live = 96;          // live 64-bit values per lane per kernel
     int occupancy = 0;          // waves per SIMD the register budget allows (512/occupancy VGPRs); 0 = auto

@@ -502,14 +502,26 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "    const u64 warm = 0ull;\n";
     }
     for (int p = 0; p < NPOP; ++p) s << "    u32 pc" << p << " = 0u;\n";
-    s << "    for (u64 r = r_begin - warm; r < r_end; r += " << U << "ull) {\n";
+    // Interleaved rows (kernels without state only): chunk cc of an instance
+    // with n chunks takes rows cc, cc + n, cc + 2n, ..., so the waves running
+    // at one time read one contiguous window of every stream rather than one
+    // distant segment each.
+    const bool inter = opt.interleave && !has_state;
+    ks.interleave = inter;
+    if (inter) {
+        s << "    const u64 istride = (u64)D.n_chunks;\n";
+        s << "    for (u64 r = cc; r < D.rows; r += istride * " << U << "ull) {\n";
+    } else {
+        s << "    for (u64 r = r_begin - warm; r < r_end; r += " << U << "ull) {\n";
+    }
     // load phase
     for (int u = 0; u < U; ++u) {
         // The row index goes through readfirstlane so that addresses are a
         // scalar row base plus the lane offset (no per-lane pointer per stream).
-        s << "      const u64 ru" << u << " = (u64)uni((u32)(r + " << u << "ull));\n";
+        const std::string rexpr = inter ? "r + " + std::to_string(u) + "ull * istride" : "r + " + std::to_string(u) + "ull";
+        s << "      const u64 ru" << u << " = (u64)uni((u32)(" << rexpr << "));\n";
         s << "      const long long w" << u << " = (long long)(ru" << u << " * 64ull + lane);\n";
-        if (U > 1) s << "      const bool ok" << u << " = r + " << u << "ull < r_end;\n";
+        if (U > 1) s << "      const bool ok" << u << " = " << rexpr << " < " << (inter ? "D.rows" : "r_end") << ";\n";
         else s << "      const bool ok" << u << " = true;\n";
         // row base (wave-uniform, scalar) + lane: loads take the SGPR-base +
         // VGPR-offset form, so no per-lane pointer is kept per stream
@@ -699,7 +711,8 @@ std::string SegGraph::key() const {
 std::string PlanOptions::key() const {
     std::ostringstream s;
     s << chunk_rows << "," << warmup_rows << "," << waves_per_block << "," << rows_per_step << "," << max_ops << ","
-      << max_live << "," << occupancy << "," << words_per_lane << "," << store_order << "," << nontemporal;
+      << max_live << "," << occupancy << "," << words_per_lane << "," << store_order << "," << nontemporal << ","
+      << interleave;
     return s.str();
 }
 

@@ -66,6 +66,7 @@ struct PlanOptions {
     int words_per_lane = 1;          // 2: lanes own word pairs in kernels without cross-lane work
     bool store_order = true;         // scheduling barrier after each output's store
     int nontemporal = 0;             // 1: nontemporal stream stores, 2: loads too
+    bool interleave = false;         // rows cc, cc+n, ... per chunk in kernels without state
     int max_ops = 4000;
     int max_live = 96;
     int occupancy = 0;               // waves per SIMD the register budget allows; 0 = from an estimate
@@ -90,6 +91,7 @@ struct KernelSpec {
     int peak_live = 0;               // most in-kernel values live at once (partitioner's count)
     int rows_per_step = 1;           // as generated (capped by register budget)
     int words_per_lane = 1;          // as generated
+    bool interleave = false;         // as generated
     int chunk_rows = 16;             // as generated
     int occupancy = 4;               // waves per SIMD the register budget was set for
     int launch_bounds = 256;         // largest workgroup the kernel accepts

@@ -1,10 +1,11 @@
 """Measured autotuner for fused plans.
 
 A schedule is a set of FusedOptions keys (chunk_rows, rows_per_step,
-waves_per_block, occupancy, words_per_lane, warmup_rows, max_ops, max_live):
-the rows a wave walks per chunk, the rows whose loads are issued together
-(words per lane in flight), the workgroup width, the register budget, 8- or
-16-byte lanes and the fusion depth.  The tuner builds one plan per candidate
+waves_per_block, occupancy, words_per_lane, interleave, warmup_rows, max_ops,
+max_live): the rows a wave walks per chunk, the rows whose loads are issued
+together (words per lane in flight), the workgroup width, the register
+budget, 8- or 16-byte lanes, contiguous or interleaved chunks and the fusion
+depth.  The tuner builds one plan per candidate
 over the same resident inputs, skips any whose compiled kernels use AGPRs or
 scratch memory (read from the code objects' metadata), checks that every
 other candidate produces the same popcounts and the same output words as the
@@ -183,14 +184,14 @@ def default_space(kind: str = "auto") -> list[dict]:
         "waves_per_block": [4],
     }
     if kind == "wide":
-        grid = {"chunk_rows": [32, 64], "rows_per_step": [1, 2], "occupancy": [1, 2, 4],
-                "waves_per_block": [4], "words_per_lane": [1, 2], "max_live": [36, 96]}
+        grid = {"chunk_rows": [16, 32, 64], "rows_per_step": [1, 2], "occupancy": [1, 2],
+                "waves_per_block": [4], "words_per_lane": [1, 2], "max_live": [36, 96], "interleave": [0, 1]}
     elif kind == "stateful":
         grid = {"chunk_rows": [4, 8, 16, 32], "rows_per_step": [2, 4, 8, 16], "occupancy": [0],
                 "waves_per_block": [1, 4]}
     elif kind == "small":
         grid = {"chunk_rows": [2, 4, 8], "rows_per_step": [2, 4, 8], "occupancy": [0, 4],
-                "waves_per_block": [2, 4]}
+                "waves_per_block": [2, 4], "interleave": [0, 1]}
     keys = sorted(grid)
     return [dict(zip(keys, v)) for v in itertools.product(*(grid[k] for k in keys))]
 
@@ -232,7 +233,7 @@ class TuneResult:
                 "worst_ms": round(max(t.ms for t in self.trials), 5)}
 
 
-_LAUNCH_KEYS = ("waves_per_block", "warmup_rows", "store_order", "nontemporal")
+_LAUNCH_KEYS = ("waves_per_block", "warmup_rows", "store_order", "nontemporal", "interleave")
 
 
 def plan_keys(program: api.Program, shape: api.Case, options_list: list[dict]) -> list[str]:

@@ -213,3 +213,51 @@ def test_random_straight_line_plans_two_words_per_lane(seed):
     ora, _ = run_model(Oracle, src, case, Oracle.SCALAR, stream_names=names[0], int_names=names[1])
     got, _ = run_plan(src, [case], WIDE_TILES[seed % len(WIDE_TILES)], names)
     assert_same(got[0], ora, case.bitlength, set(names[1]), f"seed {seed}")
+
+
+INTERLEAVED = [{"chunk_rows": 4, "rows_per_step": 1, "interleave": 1},
+               {"chunk_rows": 16, "rows_per_step": 4, "interleave": 1},
+               {"chunk_rows": 64, "rows_per_step": 1, "interleave": 1, "occupancy": 2}]
+
+
+@pytest.mark.parametrize("tile", INTERLEAVED, ids=[str(t) for t in INTERLEAVED])
+@pytest.mark.parametrize("bl", [1, 64 * 63 + 5, 64 * 64 * 7 + 64 * 3, 64 * 64 * 97 + 13])
+def test_interleaved_rows_adder(tile, bl):
+    """Chunks that take every n-th row: a last chunk with fewer rows, rows per
+    step past the stream's end, popcounts per chunk."""
+    w = workloads.adder32(bl, seed=bl + 3, width=8)
+    ora, names = oracle_outs(w.source, w.cases[0])
+    got, _ = run_plan(w.source, w.cases, tile, names)
+    assert_same(got[0], ora, bl, set(names[1]), f"adder8 {tile}")
+
+
+def test_interleaved_rows_batch_and_stateful_kernels():
+    """A batch of uneven instances (each interleaves over its own chunks),
+    and a program whose stateful kernels keep contiguous chunks."""
+    cases = [workloads.adder32(n, seed=n, width=4).cases[0] for n in (100, 64 * 64 * 2 + 1, 64 * 70, 5)]
+    src = workloads.adder32_source(4)
+    names = Oracle(src).output_names()
+    got, _ = run_plan(src, cases, INTERLEAVED[1], names)
+    for i, case in enumerate(cases):
+        ora, _ = run_model(Oracle, src, case, Oracle.SCALAR, stream_names=names[0], int_names=names[1])
+        assert_same(got[i], ora, case.bitlength, set(names[1]), f"instance {i}")
+    bl = 64 * 64 * 97 + 13
+    case = carry_case(bl, "blocks", 5)
+    ora, names = oracle_outs(CARRY_SRC, case)
+    got, _ = run_plan(CARRY_SRC, [case], dict(INTERLEAVED[1], max_ops=3, max_live=4), names)
+    assert_same(got[0], ora, bl, set(names[1]), "carry with interleave")
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_straight_line_plans_interleaved(seed):
+    src = progfuzz.random_program(200 + seed, n_stmts=40, allow_control=False, host_ints=False)
+    case = progfuzz.random_case(200 + seed, [777, 4096 * 2 + 5, 64 * 64 * 9 + 1][seed % 3])
+    case.params["P"] = case.params["P"] % 64
+    names = Oracle(src).output_names()
+    try:
+        api.compile_plan_only(api.Program(src), api.Case.from_bscase(case))
+    except Exception:
+        pytest.skip("program reads a popcount on the host")
+    ora, _ = run_model(Oracle, src, case, Oracle.SCALAR, stream_names=names[0], int_names=names[1])
+    got, _ = run_plan(src, [case], dict(INTERLEAVED[seed % len(INTERLEAVED)], max_ops=8), names)
+    assert_same(got[0], ora, case.bitlength, set(names[1]), f"seed {seed}")
