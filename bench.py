@@ -437,12 +437,14 @@ def main():
         # and the VALU bound the workload is quoted against
         line["roofline"]["planned_bytes"] = r["planned_bytes"]
         line["roofline"]["planned_frac"] = round(r["planned_bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
-        line["valu"] = {"lane_ops": r["lane_ops"], "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
-                        "bound_ms": round(r["lane_ops"] / VALU_PEAK_LANE_OPS * 1e3, 2),
-                        "frac": round(r["lane_ops"] / VALU_PEAK_LANE_OPS / (ms * 1e-3), 4)}
         line["compile_s"] = round(r["compile_s"], 1)
         line["fixup_chunks"] = r["fixup_chunks"]
         line["outputs_checked_against_perop"] = r["outputs_checked"]
+    # VALU side of the roofline: 2 32-bit lane-ops per 64-bit word per DSL op
+    lane_ops = r.get("lane_ops", 2 * r["bit_ops"] // 64)
+    line["valu"] = {"lane_ops": lane_ops, "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+                    "bound_ms": round(lane_ops / VALU_PEAK_LANE_OPS * 1e3, 4),
+                    "frac": round(lane_ops / VALU_PEAK_LANE_OPS / (ms * 1e-3), 4)}
     if r.get("tune"):
         line["autotune"] = r["tune"]
     if "perop_ms_per_step" in r:
