@@ -132,15 +132,31 @@ def collect(ctx: api.Context, specs: list[ProgramSpec], per_program: int = 16, s
     return np.array(X, dtype=np.float32), np.array(y, dtype=np.float64), np.array(gid)
 
 
-def first_clean(ctx: api.Context, source: str, case, candidates: list[dict], pred: np.ndarray,
-                processes: int = 16) -> "Measured":
-    """Compile and time candidates in predicted order until one compiles
-    within its register file (a schedule the autotuner would accept)."""
-    for k in np.argsort(pred):
-        m = measure(ctx, source, case, [candidates[int(k)]], processes=processes, allow_empty=True)
-        if m.ms:
-            return m
-    raise RuntimeError("no candidate schedule compiles within the register file")
+PICK_TOP = 3   # predicted-best candidates the model's pick compiles and times
+
+
+def model_pick(ctx: api.Context, source: str, case, candidates: list[dict], pred: np.ndarray,
+               processes: int = 16, top: int = PICK_TOP) -> "Measured":
+    """The model's choice: compile and time the `top` best-predicted
+    candidates that compile within their register file (a schedule the
+    autotuner would accept; rejected ones are replaced by the next
+    prediction) and keep the fastest.  Returns that schedule and its time,
+    with the compile and measure seconds of the whole step."""
+    order = [int(i) for i in np.argsort(pred)]
+    got = Measured()
+    pos = 0
+    while pos < len(order) and len(got.ms) < top:
+        batch = [candidates[i] for i in order[pos:pos + top - len(got.ms)]]
+        pos += len(batch)
+        m = measure(ctx, source, case, batch, processes=processes, allow_empty=True)
+        got.options += m.options
+        got.ms += m.ms
+        got.compile_s += m.compile_s
+        got.measure_s += m.measure_s
+    if not got.ms:
+        raise RuntimeError("no candidate schedule compiles within the register file")
+    b = int(np.argmin(got.ms))
+    return Measured([got.options[b]], [got.ms[b]], [], got.compile_s, got.measure_s)
 
 
 @dataclass
@@ -175,7 +191,7 @@ def evaluate(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSp
         F = costmodel.features_batch(prog, shape, candidates, max(1, (case.bitlength + 63) // 64), 1)
         pred = model.predict_ms(F)
         feat_s = time.perf_counter() - t1
-        pick = first_clean(ctx, p.source, case, candidates, pred, processes)
+        pick = model_pick(ctx, p.source, case, candidates, pred, processes)
         model_s = time.perf_counter() - t1
         # the autotuner: compile + check + time every candidate
         t0 = time.perf_counter()
@@ -238,7 +254,7 @@ def mixed_batch(ctx: api.Context, model: costmodel.CostModel, specs: list[Progra
     pick_s = time.perf_counter() - t0
     picks = []
     for i, ((p, case), pr) in enumerate(zip(built, preds)):
-        picks.append(first_clean(ctx, p.source, case, candidates, pr, processes).options[0])
+        picks.append(model_pick(ctx, p.source, case, candidates, pr, processes).options[0])
         if log:
             log(f"mixed: program {i + 1}/{len(built)} pick {picks[-1]}")
     t1 = time.perf_counter()
