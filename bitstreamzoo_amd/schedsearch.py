@@ -176,7 +176,7 @@ class EvalRow:
 
 
 def evaluate(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSpec], candidates: list[dict],
-             processes: int = 16, default: dict | None = None, log=None) -> list[EvalRow]:
+             processes: int = 16, default: dict | None = None, log=None, pick_top: int = PICK_TOP) -> list[EvalRow]:
     """Learned pick against the full measured autotune, per held-out program."""
     default = default if default is not None else {}
     rows = []
@@ -191,7 +191,7 @@ def evaluate(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSp
         F = costmodel.features_batch(prog, shape, candidates, max(1, (case.bitlength + 63) // 64), 1)
         pred = model.predict_ms(F)
         feat_s = time.perf_counter() - t1
-        pick = model_pick(ctx, p.source, case, candidates, pred, processes)
+        pick = model_pick(ctx, p.source, case, candidates, pred, processes, top=pick_top)
         model_s = time.perf_counter() - t1
         # the autotuner: compile + check + time every candidate
         t0 = time.perf_counter()
@@ -237,7 +237,7 @@ class MixedResult:
 
 def mixed_batch(ctx: api.Context, model: costmodel.CostModel, specs: list[ProgramSpec], candidates: list[dict],
                 judge_frac: float = 0.05, steps: int = 10, processes: int = 16, seed: int = 0,
-                log=None) -> MixedResult:
+                log=None, pick_top: int = PICK_TOP) -> MixedResult:
     """A batch of different generated programs run as one captured graph:
     the model picks every program's schedule from plan features, all picks
     are compiled in parallel, every program's outputs are checked against the
@@ -254,7 +254,7 @@ def mixed_batch(ctx: api.Context, model: costmodel.CostModel, specs: list[Progra
     pick_s = time.perf_counter() - t0
     picks = []
     for i, ((p, case), pr) in enumerate(zip(built, preds)):
-        picks.append(model_pick(ctx, p.source, case, candidates, pr, processes).options[0])
+        picks.append(model_pick(ctx, p.source, case, candidates, pr, processes, top=pick_top).options[0])
         if log:
             log(f"mixed: program {i + 1}/{len(built)} pick {picks[-1]}")
     t1 = time.perf_counter()
@@ -347,6 +347,8 @@ def main(argv=None) -> int:
     ap.add_argument("--max-log2-bytes", type=float, default=24)
     ap.add_argument("--collect-only", action="store_true")
     ap.add_argument("--mixed", type=int, default=0, help="run a mixed batch of N programs instead of --eval")
+    ap.add_argument("--pick-top", type=int, default=PICK_TOP,
+                    help="best-predicted candidates the model's pick compiles and times")
     a = ap.parse_args(argv)
 
     def log(m):
@@ -374,7 +376,7 @@ def main(argv=None) -> int:
     cands = sample_space(a.candidates, 12345)
     if a.mixed:
         specs = random_specs(a.mixed, a.seed + 200, nodes=(500, 5000), log2_bytes=(16, 22))
-        r = mixed_batch(ctx, model, specs, cands, processes=a.processes, seed=a.seed, log=log)
+        r = mixed_batch(ctx, model, specs, cands, processes=a.processes, seed=a.seed, log=log, pick_top=a.pick_top)
         print(json.dumps({"programs": r.n_programs, "ms_per_batch": round(r.ms, 4),
                           "default_schedules_ms": round(r.default_ms, 4), "Gbit_ops_per_s": round(r.gbitops, 1),
                           "GBps": round(r.gbps, 1), "hbm_frac": round(r.gbps / 8000.0, 4),
@@ -385,7 +387,7 @@ def main(argv=None) -> int:
         ctx.close()
         return 0
     held = random_specs(a.eval, a.seed + 100, nodes=(2000, 5000), log2_bytes=(23, 25))
-    rows = evaluate(ctx, model, held, cands, processes=a.processes, log=log)
+    rows = evaluate(ctx, model, held, cands, processes=a.processes, log=log, pick_top=a.pick_top)
     summary = {
         "train_rows": int(len(y)), "train_programs": int(len(set(gid.tolist()))), "train_s": round(model.train_seconds, 2),
         "candidates": len(cands),
