@@ -1078,7 +1078,9 @@ std::unique_ptr<FusedPlan> FusedPlan::record(hipStream_t s, const Program& prog,
     if (compile && !I.spec->kernels.empty()) {
         I.rec->jit().compile_all(*I.spec, &I.rec->compile_ms(), std::max(0, opt.compile_shard),
                                  std::max(1, opt.compile_shards));
-        I.compiled_regs = I.rec->jit().regs_of(*I.spec);
+        // a shard job (one of several processes filling the cache) does not
+        // report registers: its plan's other kernels may not exist yet
+        if (std::max(1, opt.compile_shards) == 1) I.compiled_regs = I.rec->jit().regs_of(*I.spec);
     }
     return fp;
 }
