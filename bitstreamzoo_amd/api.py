@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from .bsdata import BsCase
-from ._lib import as_u64, check, lib, ptr
+from ._lib import BszError, as_u64, check, lib, ptr  # noqa: F401  (BszError is part of the API)
 
 PEROP, FUSED = 0, 1
 

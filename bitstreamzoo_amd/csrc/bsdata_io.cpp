@@ -160,14 +160,30 @@ struct Blob {
     size_t n;
 };
 
-std::vector<uint64_t> decode_stream_value(const Json& v, const Blob& bin, size_t nw) {
+[[noreturn]] void bad_section(const std::string& key, const char* what) {
+    throw std::runtime_error("Invalid bsdata section '" + key + "': " + what);
+}
+
+// Integer field i of a section descriptor (an array of non-negative ints).
+uint64_t desc_field(const Json& r, size_t i, const std::string& key) {
+    if (r.k != Json::Arr || r.a.size() <= i) bad_section(key, "descriptor array too short");
+    const Json& x = r.a[i];
+    if (x.k != Json::Int || x.is_unsigned_big) bad_section(key, "descriptor field is not an integer");
+    const int64_t v = x.as_int(-1);
+    if (v < 0) bad_section(key, "negative offset, size or count");
+    return (uint64_t)v;
+}
+
+// [off, off + len) lies inside the binary blob (no overflow on huge offsets).
+bool in_blob(uint64_t off, uint64_t len, const Blob& bin) { return off <= bin.n && len <= bin.n - off; }
+
+std::vector<uint64_t> decode_stream_value(const Json& v, const Blob& bin, size_t nw, const std::string& key) {
     if (v.k == Json::Obj) {
         const Json* r = v.get("_bin");
-        if (r && r->k == Json::Arr && r->a.size() >= 2) {
-            size_t off = (size_t)r->a[0].as_int(), nb = (size_t)r->a[1].as_int();
-            if (off + nb <= bin.n) return bytes_to_words(bin.p + off, nb, nw);
-        }
-        return std::vector<uint64_t>(nw, 0);
+        if (!r) return std::vector<uint64_t>(nw, 0);
+        const uint64_t off = desc_field(*r, 0, key), nb = desc_field(*r, 1, key);
+        if (!in_blob(off, nb, bin)) bad_section(key, "_bin range lies outside the binary part");
+        return bytes_to_words(bin.p + off, nb, nw);
     }
     return json_value_words(v, nw);
 }
@@ -179,31 +195,27 @@ void decode_section(const Json& sec, const Blob& bin, size_t nw, CaseData& c, bo
         const Json& v = kv.second;
         if (v.k == Json::Obj && v.has("_broadcast_pack")) {
             const Json& r = *v.get("_broadcast_pack");
-            int64_t count = r.a.size() > 0 ? r.a[0].as_int() : 0;
-            size_t off = r.a.size() > 1 ? (size_t)r.a[1].as_int() : 0;
-            std::vector<bool> bv((size_t)std::max<int64_t>(0, count));
-            for (int64_t i = 0; i < count; ++i) {
-                size_t bi = off + (size_t)(i / 8);
-                bv[i] = bi < bin.n && ((bin.p[bi] >> (i % 8)) & 1);
-            }
+            const uint64_t count = desc_field(r, 0, k), off = desc_field(r, 1, k);
+            if (!in_blob(off, (count + 7) / 8, bin)) bad_section(k, "_broadcast_pack bits lie outside the binary part");
+            std::vector<bool> bv((size_t)count);
+            for (uint64_t i = 0; i < count; ++i) bv[i] = (bin.p[off + i / 8] >> (i % 8)) & 1;
             c.broadcast[k] = std::move(bv);
         } else if (v.k == Json::Obj && v.has("_array_bin")) {
             const Json& r = *v.get("_array_bin");
-            int64_t count = r.a[0].as_int();
-            size_t off = (size_t)r.a[1].as_int(), stride = (size_t)r.a[2].as_int();
+            const uint64_t count = desc_field(r, 0, k), off = desc_field(r, 1, k), stride = desc_field(r, 2, k);
+            if (count > (1ull << 31)) bad_section(k, "_array_bin count too large");
+            if (count && (stride > bin.n || !in_blob(off, count * stride, bin)))
+                bad_section(k, "_array_bin elements lie outside the binary part");
             auto& arr = c.input_arrays[k];
-            for (int64_t i = 0; i < count; ++i) {
-                size_t st = off + (size_t)i * stride;
-                arr[i] = st + stride <= bin.n ? bytes_to_words(bin.p + st, stride, nw) : std::vector<uint64_t>(nw, 0);
-            }
+            for (uint64_t i = 0; i < count; ++i) arr[(int64_t)i] = bytes_to_words(bin.p + off + i * stride, stride, nw);
         } else if (v.k == Json::Obj && v.has("_bin")) {
-            if (scalars) c.inputs[k] = decode_stream_value(v, bin, nw);
+            if (scalars) c.inputs[k] = decode_stream_value(v, bin, nw, k);
         } else if (v.k == Json::Arr) {
             auto& arr = c.input_arrays[k];
             for (size_t i = 0; i < v.a.size(); ++i) arr[(int64_t)i] = json_value_words(v.a[i], nw);
         } else if (v.k == Json::Obj) {
             auto& arr = c.input_arrays[k];
-            for (auto& e : v.o) arr[std::stoll(e.first)] = decode_stream_value(e.second, bin, nw);
+            for (auto& e : v.o) arr[std::stoll(e.first)] = decode_stream_value(e.second, bin, nw, k);
         } else if (scalars) {
             c.inputs[k] = json_value_words(v, nw);
         }

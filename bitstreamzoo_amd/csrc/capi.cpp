@@ -26,6 +26,8 @@ struct bsz_ctx {
     size_t add_state_bytes = 0;
     unsigned long long* slot = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    bool capturing = false;          // between bsz_capture_begin and bsz_capture_end
+    int64_t capture_launch0 = 0;     // kernel_launch_count() at bsz_capture_begin
 };
 
 struct bsz_program {
@@ -77,6 +79,14 @@ int guard(bsz_ctx* ctx, F&& f) {
 }
 
 Geom geom_of(int64_t bitlength) { return Geom::make(bitlength, 1); }
+
+// An op enqueued while the context captures a graph: refused once the graph
+// would exceed kMaxGraphLaunches launches (each op launches at most one kernel).
+void capture_gate(bsz_ctx* ctx) {
+    if (ctx->capturing && kernel_launch_count() - ctx->capture_launch0 >= kMaxGraphLaunches)
+        throw RunError("graph capture refused: more than " + std::to_string(kMaxGraphLaunches) +
+                       " kernel launches in one graph");
+}
 
 void ensure_scalar_state(bsz_ctx* ctx, const Geom& g) {
     size_t need = AddScanState::bytes(g);
@@ -165,26 +175,27 @@ int bsz_d2h(bsz_ctx* ctx, uint64_t* host, const uint64_t* dev, uint64_t n) {
 }
 
 int bsz_op_fill(bsz_ctx* ctx, uint64_t* dst, int ones, int64_t bl) {
-    return guard(ctx, [&] { launch_fill(ctx->s, dst, ones != 0, geom_of(bl)); });
+    return guard(ctx, [&] { capture_gate(ctx); launch_fill(ctx->s, dst, ones != 0, geom_of(bl)); });
 }
 int bsz_op_copy(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, int64_t bl) {
-    return guard(ctx, [&] { launch_pointwise(ctx->s, PwOp::Copy, dst, src, nullptr, geom_of(bl)); });
+    return guard(ctx, [&] { capture_gate(ctx); launch_pointwise(ctx->s, PwOp::Copy, dst, src, nullptr, geom_of(bl)); });
 }
 int bsz_op_not(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, int64_t bl) {
-    return guard(ctx, [&] { launch_pointwise(ctx->s, PwOp::Not, dst, a, nullptr, geom_of(bl)); });
+    return guard(ctx, [&] { capture_gate(ctx); launch_pointwise(ctx->s, PwOp::Not, dst, a, nullptr, geom_of(bl)); });
 }
 int bsz_op_and(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bl) {
-    return guard(ctx, [&] { launch_pointwise(ctx->s, PwOp::And, dst, a, b, geom_of(bl)); });
+    return guard(ctx, [&] { capture_gate(ctx); launch_pointwise(ctx->s, PwOp::And, dst, a, b, geom_of(bl)); });
 }
 int bsz_op_or(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bl) {
-    return guard(ctx, [&] { launch_pointwise(ctx->s, PwOp::Or, dst, a, b, geom_of(bl)); });
+    return guard(ctx, [&] { capture_gate(ctx); launch_pointwise(ctx->s, PwOp::Or, dst, a, b, geom_of(bl)); });
 }
 int bsz_op_xor(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bl) {
-    return guard(ctx, [&] { launch_pointwise(ctx->s, PwOp::Xor, dst, a, b, geom_of(bl)); });
+    return guard(ctx, [&] { capture_gate(ctx); launch_pointwise(ctx->s, PwOp::Xor, dst, a, b, geom_of(bl)); });
 }
 int bsz_op_add(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bl) {
     return guard(ctx, [&] {
         if (dst == a || dst == b) throw std::invalid_argument("bsz_op_add: dst aliases an operand");
+        capture_gate(ctx);
         Geom g = geom_of(bl);
         ensure_scalar_state(ctx, g);
         launch_add(ctx->s, dst, a, b, g, ctx->add_state);
@@ -193,12 +204,14 @@ int bsz_op_add(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b
 int bsz_op_shl(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, uint64_t k, int64_t bl) {
     return guard(ctx, [&] {
         if (dst == src) throw std::invalid_argument("bsz_op_shl: dst aliases src");
+        capture_gate(ctx);
         launch_shl(ctx->s, dst, src, k, geom_of(bl));
     });
 }
 int bsz_op_shr(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, uint64_t k, int64_t bl) {
     return guard(ctx, [&] {
         if (dst == src) throw std::invalid_argument("bsz_op_shr: dst aliases src");
+        capture_gate(ctx);
         launch_shr(ctx->s, dst, src, k, geom_of(bl));
     });
 }
@@ -460,16 +473,22 @@ int bsz_timer_stop(bsz_ctx* ctx, double* ms) {
 }
 
 int bsz_capture_begin(bsz_ctx* ctx) {
-    return guard(ctx, [&] { BSZ_HIP(hipStreamBeginCapture(ctx->s, hipStreamCaptureModeThreadLocal)); });
+    return guard(ctx, [&] {
+        BSZ_HIP(hipStreamBeginCapture(ctx->s, hipStreamCaptureModeThreadLocal));
+        ctx->capturing = true;
+        ctx->capture_launch0 = kernel_launch_count();
+    });
 }
 
 int bsz_capture_end(bsz_ctx* ctx, void** graph_exec) {
     return guard(ctx, [&] {
-        hipGraph_t g;
-        hipGraphExec_t ex;
+        ctx->capturing = false;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ex = nullptr;
         BSZ_HIP(hipStreamEndCapture(ctx->s, &g));
-        BSZ_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
+        if (e != hipSuccess) throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         *graph_exec = ex;
     });
 }

@@ -114,4 +114,14 @@ void launch_copy_words(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64
 
 int64_t kernel_launch_count();
 
+// Kernel launches one captured hipGraph may hold.  Capturing 131k per-op
+// launches (2048 Shift-And instances) into one graph ended in a host segfault
+// on MI355X / ROCm 7 whose cause was not found; graphs of up to 20k launches
+// have worked, so larger captures are refused with an error.
+constexpr int64_t kMaxGraphLaunches = 20000;
+
+// Ends a capture on `s` that is still open and drops its partial graph, so a
+// failed capture never leaves the stream in capture mode.
+void abandon_capture(hipStream_t s);
+
 }  // namespace bsz

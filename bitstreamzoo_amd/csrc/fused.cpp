@@ -108,14 +108,35 @@ std::string default_arch() {
     return "gfx950";
 }
 
+// hipRTC options of every generated unit (after --offload-arch).  Packed
+// FP32 instructions stay disabled, as in the rest of the library.
+const std::vector<std::string>& rtc_options() {
+    static const std::vector<std::string> o = {"-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                                               "-packed-fp32-ops"};
+    return o;
+}
+
+// Identifies the compiler and its options in the code-object cache key, so a
+// ROCm upgrade or a changed flag never reuses an old code object.
+const std::string& rtc_identity() {
+    static const std::string id = [] {
+        int major = 0, minor = 0;
+        (void)hiprtcVersion(&major, &minor);
+        std::string s = "hiprtc " + std::to_string(major) + "." + std::to_string(minor) + " |";
+        for (const std::string& o : rtc_options()) s += " " + o;
+        return s;
+    }();
+    return id;
+}
+
 std::vector<char> compile_module(const std::string& src, const std::string& arch, std::string* log) {
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "bsz_fused.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
         throw RunError("hiprtcCreateProgram failed");
     std::string archopt = "--offload-arch=" + arch;
-    const char* opts[] = {archopt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-                          "-packed-fp32-ops"};
-    hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    std::vector<const char*> opts{archopt.c_str()};
+    for (const std::string& o : rtc_options()) opts.push_back(o.c_str());
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
     size_t logsz = 0;
     hiprtcGetProgramLogSize(prog, &logsz);
     std::string lg(logsz, '\0');
@@ -187,6 +208,33 @@ KernelRegs kernel_regs(const std::vector<char>& code) {
     r.agprs = (int)msgpack_uint_after(code, ".agpr_count");
     r.scratch = (int)msgpack_uint_after(code, ".private_segment_fixed_size");
     return r;
+}
+
+// A generated kernel runs only if its code object shows that every value stays
+// in the architectural VGPRs: no AGPRs, no scratch, and all three counts
+// present in the metadata (a missing count is treated as a refusal).  Two wide
+// random-DAG partitions whose kernels overflowed into AGPRs (one also into
+// scratch) produced 64-bit values with correct low and wrong high halves on
+// gfx950; the cause inside the compiled code was not found, so such kernels
+// are refused rather than run.
+bool regs_clean(const KernelRegs& r) { return r.vgprs > 0 && r.agprs == 0 && r.scratch == 0; }
+
+void refuse_unclean(const PlanSpec& p, const std::vector<KernelRegs>& regs) {
+    for (size_t k = 0; k < regs.size() && k < p.kernels.size(); ++k) {
+        const KernelRegs& r = regs[k];
+        if (regs_clean(r)) continue;
+        std::ostringstream m;
+        m << "schedule refused: generated kernel " << p.kernels[k].name << " (" << p.kernels[k].nodes.size()
+          << " nodes, " << p.kernels[k].inputs.size() << " inputs) ";
+        if (r.vgprs < 0 || r.agprs < 0 || r.scratch < 0)
+            m << "has a code object without register metadata";
+        else
+            m << "uses " << r.agprs << " AGPRs and " << r.scratch << " bytes of scratch per lane beside "
+              << r.vgprs << " VGPRs";
+        m << "; kernels that overflow the architectural VGPRs are not run (choose a higher occupancy, fewer "
+             "rows_per_step or a smaller max_live)";
+        throw RunError(m.str());
+    }
 }
 
 // The entry points of a plan's kernels (code objects owned by the Jit).
@@ -286,7 +334,10 @@ public:
                 hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, it->second.second) == hipSuccess
                     ? (size_t)std::max(0, local)
                     : 0);
+            // the loader's view of private memory must agree with the metadata
+            if (m->scratch.back() > 0 && m->regs.back().scratch == 0) m->regs.back().scratch = (int)m->scratch.back();
         }
+        refuse_unclean(p, m->regs);
         return m;
     }
 
@@ -297,7 +348,12 @@ private:
     std::unordered_map<uint64_t, std::pair<hipModule_t, hipFunction_t>> loaded_;    // on the device
     std::unordered_map<uint64_t, KernelRegs> regs_;
 
-    static uint64_t unit_hash(const std::string& unit, const std::string& arch) { return source_hash(unit + "|" + arch); }
+    static uint64_t unit_hash(const std::string& unit, const std::string& arch) {
+        return source_hash(unit + "|" + arch + "|" + rtc_identity());
+    }
+    // A cache file is a 16-byte header (magic, the unit's key) and the code
+    // object; a file whose header does not match its name is not used.
+    static constexpr char kMagic[8] = {'B', 'S', 'Z', 'C', 'O', '0', '0', '2'};
     std::string path_of(uint64_t h) const {
         char name[64];
         std::snprintf(name, sizeof name, "/%016llx.co", (unsigned long long)h);
@@ -317,8 +373,7 @@ private:
             std::lock_guard<std::mutex> g(mu_);
             if (codes_.count(h)) return true;
         }
-        std::ifstream f(path_of(h), std::ios::binary);
-        return (bool)f && f.peek() != std::ifstream::traits_type::eof();
+        return !read_code(h).empty();
     }
     std::vector<char> read_code(uint64_t h) {
         {
@@ -328,7 +383,12 @@ private:
         }
         std::ifstream f(path_of(h), std::ios::binary);
         if (!f) return {};
-        return std::vector<char>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+        std::vector<char> all((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+        uint64_t key = 0;
+        if (all.size() <= 16 || std::memcmp(all.data(), kMagic, 8) != 0) return {};
+        std::memcpy(&key, all.data() + 8, 8);
+        if (key != h) return {};
+        return std::vector<char>(all.begin() + 16, all.end());
     }
     void store(uint64_t h, const std::vector<char>& code) {
         {
@@ -341,6 +401,8 @@ private:
         tmpn << path << ".tmp" << (long)getpid() << "_" << std::this_thread::get_id();
         {
             std::ofstream f(tmpn.str(), std::ios::binary);
+            f.write(kMagic, 8);
+            f.write(reinterpret_cast<const char*>(&h), 8);
             f.write(code.data(), (std::streamsize)code.size());
         }
         std::rename(tmpn.str().c_str(), path.c_str());
@@ -359,6 +421,32 @@ struct Arena {
         return o;
     }
 };
+
+// Records the work `enqueue` puts on `s` into an executable graph.  Refuses a
+// run of more than kMaxGraphLaunches launches before capturing anything, and
+// ends the capture if `enqueue` throws, so the stream never stays in capture
+// mode.
+template <class F>
+hipGraphExec_t capture_graph(hipStream_t s, int64_t launches, F&& enqueue) {
+    if (launches > kMaxGraphLaunches)
+        throw RunError("graph capture of " + std::to_string(launches) + " kernel launches refused (at most " +
+                       std::to_string(kMaxGraphLaunches) +
+                       " per graph: a 131k-launch capture crashed the host); run such a plan without capture");
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    BSZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue();
+    } catch (...) {
+        abandon_capture(s);
+        throw;
+    }
+    BSZ_HIP(hipStreamEndCapture(s, &g));
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return ex;
+}
 
 class PlanRT {
 public:
@@ -520,14 +608,19 @@ public:
         per_run_launches_ = launches_ - before;
     }
 
+    // Launches of one run: each kernel, a fix-up per stateful kernel, and the
+    // epoch kernel when the last kernel has no fix-up.
+    int64_t launches_per_run() const {
+        const size_t K = spec->kernels.size();
+        int64_t n = (int64_t)K;
+        for (const KRes& r : kres_) n += r.nst ? 1 : 0;
+        if (K == 0 || !kres_[K - 1].nst) ++n;
+        return n;
+    }
+
     void capture(hipStream_t s) {
         if (graph_) { (void)hipGraphExecDestroy(graph_); graph_ = nullptr; }
-        hipGraph_t g;
-        BSZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        launch(s);
-        BSZ_HIP(hipStreamEndCapture(s, &g));
-        BSZ_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(g);
+        graph_ = capture_graph(s, launches_per_run(), [&] { launch(s); });
     }
 
     void replay(hipStream_t s) {
@@ -1080,7 +1173,10 @@ std::unique_ptr<FusedPlan> FusedPlan::record(hipStream_t s, const Program& prog,
                                  std::max(1, opt.compile_shards));
         // a shard job (one of several processes filling the cache) does not
         // report registers: its plan's other kernels may not exist yet
-        if (std::max(1, opt.compile_shards) == 1) I.compiled_regs = I.rec->jit().regs_of(*I.spec);
+        if (std::max(1, opt.compile_shards) == 1) {
+            I.compiled_regs = I.rec->jit().regs_of(*I.spec);
+            refuse_unclean(*I.spec, I.compiled_regs);
+        }
     }
     return fp;
 }
@@ -1234,14 +1330,9 @@ void FusedPlan::capture() {
     Impl& I = *impl_;
     if (I.executor == 0) {
         if (graph_) { (void)hipGraphExecDestroy((hipGraphExec_t)graph_); graph_ = nullptr; }
-        hipGraph_t gr;
-        hipGraphExec_t ex;
-        BSZ_HIP(hipStreamBeginCapture(I.s, hipStreamCaptureModeThreadLocal));
-        run();
-        BSZ_HIP(hipStreamEndCapture(I.s, &gr));
-        BSZ_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(gr);
-        graph_ = ex;
+        const int64_t pops = I.spec->g.pops.empty() ? 0 : 1 + (int64_t)I.spec->g.pops.size();
+        const int64_t per_inst = (int64_t)I.perop_order.size() + pops;
+        graph_ = capture_graph(I.s, per_inst * (int64_t)I.geoms.size(), [&] { run(); });
         return;
     }
     if (!I.spec->kernels.empty()) I.rt->capture(I.s);

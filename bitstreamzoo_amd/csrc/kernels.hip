@@ -356,6 +356,14 @@ void launch_copy_words(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64
     check_launch();
 }
 
+void abandon_capture(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess || st == hipStreamCaptureStatusNone) return;
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+}
+
 // ── DevPool ──
 
 DevPool::~DevPool() { trim(); }

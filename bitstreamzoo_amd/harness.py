@@ -20,18 +20,18 @@ from dataclasses import dataclass
 
 from ._lib import CLI_PATH
 
-# glibc's memset switches to `rep stosb` above this many bytes; the host
-# buffers of large streams stay on the vectorized path below it (the harness
-# sets the same tunable for its C++ backends, benchmark/base.py:84-95).
-_REP_STOSB_THRESHOLD = 2_000_000_000
-
-
-def backend_env() -> dict:
-    env = os.environ.copy()
-    tunable = f"glibc.cpu.x86_rep_stosb_threshold={_REP_STOSB_THRESHOLD}"
-    cur = env.get("GLIBC_TUNABLES", "")
-    if "x86_rep_stosb_threshold" not in cur:
-        env["GLIBC_TUNABLES"] = f"{cur}:{tunable}" if cur else tunable
+def subprocess_env() -> dict:
+    """The environment bsim_hip runs in: this process's, plus one glibc
+    tuning knob unless the caller already set it.  Large host buffers are
+    cleared by memset, which on some AMD Zen 4 hosts is slow once glibc
+    switches it to `rep stosb`; raising that switch point to 2e9 bytes keeps
+    every stream buffer on the vector path.  The reference harness applies
+    the same knob to its C++ backends (benchmark/base.py:84-95)."""
+    knob, value = "glibc.cpu.x86_rep_stosb_threshold", 2 * 10**9
+    env = dict(os.environ)
+    have = env.get("GLIBC_TUNABLES", "")
+    if knob.rsplit(".", 1)[-1] not in have:
+        env["GLIBC_TUNABLES"] = ":".join(x for x in (have, f"{knob}={value}") if x)
     return env
 
 
@@ -121,11 +121,11 @@ class HipBackend:
         cmd = self._cmd(prog)
         if bsdata_path:
             proc = subprocess.run(cmd + ["--input", bsdata_path], capture_output=True, text=True,
-                                  timeout=self.timeout, env=backend_env())
+                                  timeout=self.timeout, env=subprocess_env())
         else:
             case = build_input_json(inputs, params, input_arrays, bitlength, iterations, feedback)
             proc = subprocess.run(cmd, input=json.dumps(case), capture_output=True, text=True,
-                                  timeout=self.timeout, env=backend_env())
+                                  timeout=self.timeout, env=subprocess_env())
         self._check(proc)
         out = json.loads(proc.stdout)
         if isinstance(out, list):  # a .bsdata file with several cases: the first one
@@ -138,7 +138,7 @@ class HipBackend:
         results = []
         for p in bsdata_paths:
             proc = subprocess.run(self._cmd(prog) + ["--input", p], capture_output=True, text=True,
-                                  timeout=self.timeout, env=backend_env())
+                                  timeout=self.timeout, env=subprocess_env())
             self._check(proc)
             out = json.loads(proc.stdout)
             results.extend(out if isinstance(out, list) else [out])

@@ -210,11 +210,20 @@ class Trial:
 def register_clean(info: dict) -> bool:
     """Every compiled kernel of the plan stays in its architectural VGPRs
     (no AGPR use, no scratch); kernels that overflowed have given wrong
-    results, so the tuner does not use such schedules."""
+    results, so the tuner does not use such schedules.  A kernel whose
+    counts are missing or unknown (-1) counts as unclean.  (The library
+    itself refuses such plans; this is the tuner's own check of the info.)"""
     for k in info.get("kernels", []):
-        if k.get("agprs", 0) > 0 or k.get("scratch_bytes", 0) > 0:
+        vg, ag, sc = k.get("vgprs", -1), k.get("agprs", -1), k.get("scratch_bytes", -1)
+        if vg is None or ag is None or sc is None or vg <= 0 or ag != 0 or sc != 0:
             return False
     return True
+
+
+def refused(e: Exception) -> bool:
+    """The library refused a schedule because its compiled kernels overflow
+    the architectural VGPRs (a RUNTIME error naming the refusal)."""
+    return isinstance(e, api.BszError) and e.code == -3 and "schedule refused" in str(e)
 
 
 @dataclass
@@ -272,7 +281,15 @@ def autotune(ctx: api.Context, program: api.Program, residents: list[Resident], 
         opts = dict(base_options or {})
         opts.update(cand)
         tc = time.perf_counter()
-        plans = [api.Plan(ctx, program, shape, opts) for _ in residents]
+        try:
+            plans = [api.Plan(ctx, program, shape, opts) for _ in residents]
+        except api.BszError as e:
+            if not refused(e):
+                raise
+            res.skipped.append((cand, "kernels use AGPRs or scratch"))
+            if log:
+                log(f"tune {cand}: skipped (the library refused it: compiled kernels use AGPRs or scratch)")
+            continue
         compile_s = time.perf_counter() - tc
         try:
             for p, r in zip(plans, residents):

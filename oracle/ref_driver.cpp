@@ -12,6 +12,7 @@
 // Used by tests to pin the oracle (oracle/bsz_oracle.cpp) against the real
 // reference on programs and inputs the committed fixtures do not cover.
 
+#include "ast_liveness.h"
 #include "cpu_backend.h"
 #include "interpreter.h"
 #include "parser.h"
@@ -108,6 +109,10 @@ int bsref_run(bsref_case* c, int variant, int add_mode, int reuse_mem, char* err
 }
 
 int64_t bsref_op_count(const bsref_case* c) { return c->res.op_count; }
+
+// The reference's static op count (count_ops_ast, ast_liveness.cpp:723-727)
+// of the parsed program under the case's params; needs no run.
+int64_t bsref_static_ops(const bsref_case* c) { return count_ops_ast(c->prog, c->params); }
 double bsref_exec_ms(const bsref_case* c) { return c->exec_ms; }
 
 static void out_masked(const bsref_case* c, const std::vector<uint64_t>& v, uint64_t* out, uint64_t n) {

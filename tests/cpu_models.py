@@ -226,6 +226,12 @@ class RefInterp(_Model):
     def _run(self, err, variant=2, add_mode=0, reuse_mem=1):
         return self.L.bsref_run(self.h, variant, add_mode, reuse_mem, err, 4096)
 
+    def static_ops(self) -> int:
+        """The reference's count_ops_ast under the loaded params (no run)."""
+        self.L.bsref_static_ops.restype = C.c_int64
+        self.L.bsref_static_ops.argtypes = [C.c_void_p]
+        return self.L.bsref_static_ops(self.h)
+
 
 def oracle_words_op(fn: str, *arrays, k: int = 0, variant: int = 0):
     """Call one of the oracle's exported word kernels."""

@@ -157,3 +157,48 @@ def test_string_results_come_from_one_complete_call():
         return len(t)
 
     assert _fill_str(call, cap=4) == '{"a":"' + "x" * 70000 + '"}'
+
+
+def test_schedule_that_overflows_vgprs_is_refused():
+    """Two rows of the adder's 65 inputs in flight at occupancy 1 compile to a
+    kernel that uses AGPRs; such kernels gave wrong high halves of 64-bit
+    values on MI355X, so the library refuses the schedule (RUNTIME error)
+    instead of running it.  The same adder at occupancy 2 is accepted."""
+    w = workloads.adder32(1 << 16, seed=11)
+    prog, case = api.Program(w.source), api.Case.from_bscase(w.cases[0])
+    with pytest.raises(BszError) as e:
+        api.compile_plan_only(prog, case, {"chunk_rows": 8, "rows_per_step": 2, "occupancy": 1})
+    assert e.value.code == -3 and "schedule refused" in str(e.value) and "AGPRs" in str(e.value)
+    info = api.compile_plan_only(prog, case, {"chunk_rows": 4, "rows_per_step": 1, "occupancy": 2})
+    assert all(k["agprs"] == 0 and k["scratch_bytes"] == 0 and k["vgprs"] > 0 for k in info["kernels"])
+
+
+def test_register_check_fails_closed():
+    """The tuner's own check treats missing or unknown register counts as
+    unclean, so a kernel without metadata is never timed or picked."""
+    from bitstreamzoo_amd import tuning
+    good = {"vgprs": 120, "agprs": 0, "scratch_bytes": 0}
+    assert tuning.register_clean({"kernels": [good]})
+    for bad in ({"vgprs": 120, "agprs": 4, "scratch_bytes": 0}, {"vgprs": 120, "agprs": 0, "scratch_bytes": 16},
+                {"vgprs": -1, "agprs": -1, "scratch_bytes": -1}, {"vgprs": 120}, {}):
+        assert not tuning.register_clean({"kernels": [good, bad]}), bad
+
+
+def test_code_object_cache_rejects_foreign_files(tmp_path):
+    """Cached code objects carry a header with their key (compiler version,
+    options, arch and source); a file whose header is missing or wrong is
+    ignored and recompiled, never loaded."""
+    import glob
+    opts = {"cache_dir": str(tmp_path)}
+    w = workloads.matchstar(256)
+    prog, case = api.Program(w.source), api.Case.from_bscase(w.cases[0])
+    api.compile_plan_only(prog, case, opts)
+    files = glob.glob(str(tmp_path / "*.co"))
+    assert len(files) == 1
+    good = open(files[0], "rb").read()
+    assert good[:8] == b"BSZCO002"
+    with open(files[0], "wb") as f:
+        f.write(good[16:])                      # a bare code object: no header
+    info = api.compile_plan_only(prog, case, opts)
+    assert info["kernels"][0]["vgprs"] > 0
+    assert open(files[0], "rb").read() == good  # recompiled and rewritten

@@ -123,3 +123,90 @@ def test_cli_reads_every_reference_fixture():
                     assert int(dj["input_arrays"][k][str(i)], 16) == int.from_bytes(w.tobytes(), "little"), (cid, k, i)
         n += 1
     assert n > 50
+
+
+@pytest.mark.parametrize("section,label", [
+    ({"x": {"_bin": [1 << 62, 16]}}, "_bin offset near 2^64"),
+    ({"x": {"_bin": [0, 1 << 40]}}, "_bin longer than the file"),
+    ({"x": {"_bin": [0]}}, "_bin descriptor too short"),
+    ({"x": {"_bin": [-8, 8]}}, "negative offset"),
+    ({"x": {"_bin": ["0", 8]}}, "offset not an integer"),
+    ({"a": {"_array_bin": [2, 0]}}, "_array_bin descriptor too short"),
+    ({"a": {"_array_bin": [3, 0, 8]}}, "_array_bin past the end"),
+    ({"a": {"_array_bin": [2, 8, (1 << 63) - 1]}}, "_array_bin stride overflow"),
+    ({"a": {"_broadcast_pack": [1 << 20, 0, 1]}}, "_broadcast_pack past the end"),
+])
+def test_cli_rejects_malformed_bsdata(tmp_path, section, label):
+    """Section descriptors that are short, negative, non-integer or point
+    outside the binary part are parse errors (exit 1, 'Error reading bsdata'),
+    never out-of-bounds reads; offsets near 2^64 cannot wrap the check."""
+    header = {"bitlength": 64, "inputs": {k: v for k, v in section.items() if "_array_bin" not in v
+                                          and "_broadcast_pack" not in v},
+              "input_arrays": {k: v for k, v in section.items() if "_array_bin" in v or "_broadcast_pack" in v}}
+    path = tmp_path / "bad.bsdata"
+    path.write_bytes(json.dumps(header).encode() + b"\0" + bytes(16))
+    r = subprocess.run([_lib.CLI_PATH, "/dev/null", "--input", str(path), "--dump-input"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1, (label, r.stdout[:200])
+    assert "Error reading bsdata" in r.stderr and "Invalid bsdata section" in r.stderr, (label, r.stderr)
+
+
+def test_cli_reads_well_formed_sections_at_the_blob_edge(tmp_path):
+    """A _bin / _array_bin / _broadcast_pack whose range ends exactly at the
+    end of the binary part is read in full."""
+    blob = bytes(range(16)) + bytes([0b101])
+    header = {"bitlength": 64, "inputs": {"x": {"_bin": [8, 8]}},
+              "input_arrays": {"a": {"_array_bin": [2, 0, 8]}, "b": {"_broadcast_pack": [3, 16, 1]}}}
+    path = tmp_path / "edge.bsdata"
+    path.write_bytes(json.dumps(header).encode() + b"\0" + blob)
+    r = subprocess.run([_lib.CLI_PATH, "/dev/null", "--input", str(path), "--dump-input"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    assert int(d["inputs"]["x"], 16) == int.from_bytes(blob[8:16], "little")
+    assert int(d["input_arrays"]["a"]["1"], 16) == int.from_bytes(blob[8:16], "little")
+
+
+def _static_ops_three_ways(src, bscase):
+    from cpu_models import RefInterp
+    ref = RefInterp(src).load(bscase).static_ops()
+    o = Oracle(src).load(bscase)
+    try:
+        o.run(Oracle.SCALAR)
+    except RuntimeError:
+        pass  # a runtime error after the static count does not matter here
+    return ref, o.static_ops(), api.Program(src).static_ops(api.Case.from_bscase(bscase))
+
+
+@pytest.mark.skipif(not ref_fixtures.available(), reason="reference tree not present")
+def test_static_ops_pinned_to_reference_on_every_fixture():
+    """count_ops_ast of the reference itself (oracle/_ref, ast_liveness.cpp:
+    629-727) equals the oracle's and the product's static op count on every
+    committed fixture program under its own params (for bodies counted times
+    the trip count, if/while bodies once)."""
+    from cpu_models import RefInterp
+    if not RefInterp.available():
+        pytest.skip("oracle/_ref not built")
+    n = 0
+    for cid, bs, path in ref_fixtures.unit_cases():
+        ref, ora, prod = _static_ops_three_ways(open(bs).read(), read_cases(path)[0])
+        assert ref == ora == prod, (cid, ref, ora, prod)
+        n += 1
+    assert n >= 100
+
+
+@pytest.mark.skipif(not ref_fixtures.available(), reason="reference tree not present")
+def test_static_ops_pinned_to_reference_on_control_flow_programs():
+    """The same on random programs with nested for / if / while, arrays and
+    popcount assignments, where a body counted once versus per iteration
+    would differ."""
+    from cpu_models import RefInterp
+    if not RefInterp.available():
+        pytest.skip("oracle/_ref not built")
+    seen_control = 0
+    for seed in range(80):
+        src = progfuzz.random_program(seed, n_stmts=30, allow_control=True)
+        ref, ora, prod = _static_ops_three_ways(src, progfuzz.random_case(seed, 128))
+        assert ref == ora == prod, (seed, ref, ora, prod, src)
+        seen_control += ("if (" in src) or ("while" in src)
+    assert seen_control >= 20
