@@ -79,6 +79,7 @@ _SIGS = {
     "bsz_plan_replay": (C.c_int, [vp, C.c_int]),
     "bsz_plan_read_int": (C.c_int, [vp, C.c_int, C.c_char_p, i64p]),
     "bsz_plan_mismatches": (C.c_int, [vp, C.c_int, i64p]),
+    "bsz_plan_unproven": (C.c_int, [vp, C.c_int, i64p]),
     "bsz_plan_info": (C.c_int64, [vp, C.c_char_p, C.c_uint64]),
     "bsz_plan_source": (C.c_int64, [vp, C.c_char_p, C.c_uint64]),
     "bsz_plan_set_executor": (C.c_int, [vp, C.c_int]),

@@ -352,6 +352,13 @@ class Plan:
         check(lib().bsz_plan_mismatches(self.h, kernel, C.byref(v)), self.ctx.h)
         return v.value
 
+    def unproven(self, kernel: int = 0) -> int:
+        """Chunks of `kernel` whose warm-up did not prove their start state
+        exact in the last run."""
+        v = C.c_int64()
+        check(lib().bsz_plan_unproven(self.h, kernel, C.byref(v)), self.ctx.h)
+        return v.value
+
     def info(self) -> dict:
         return json.loads(_get_str(lib().bsz_plan_info, self.h))
 

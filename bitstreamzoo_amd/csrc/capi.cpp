@@ -96,7 +96,7 @@ void ensure_scalar_state(bsz_ctx* ctx, const Geom& g) {
         BSZ_HIP(hipMemsetAsync(ctx->add_state, 0, need, ctx->s));
         ctx->add_state_bytes = need;
     }
-    if (!ctx->slot) BSZ_HIP(hipMalloc(&ctx->slot, 64));
+    if (!ctx->slot) BSZ_HIP(hipMalloc(&ctx->slot, 8 * kPopSlotWords));
 }
 
 }  // namespace
@@ -219,12 +219,9 @@ int bsz_op_popcount(bsz_ctx* ctx, const uint64_t* src, int64_t bl, int64_t* out)
     return guard(ctx, [&] {
         Geom g = geom_of(bl);
         ensure_scalar_state(ctx, g);
-        launch_zero_u64(ctx->s, ctx->slot, 1);
+        launch_zero_u64(ctx->s, ctx->slot, kPopSlotWords);
         launch_popcount(ctx->s, ctx->slot, src, g);
-        unsigned long long v = 0;
-        BSZ_HIP(hipMemcpyAsync(&v, ctx->slot, 8, hipMemcpyDeviceToHost, ctx->s));
-        BSZ_HIP(hipStreamSynchronize(ctx->s));
-        *out = (int64_t)v;
+        *out = (int64_t)read_popcount_slots(ctx->s, ctx->slot);
     });
 }
 int bsz_op_is_nonzero(bsz_ctx* ctx, const uint64_t* src, int64_t bl, int* out) {
@@ -446,6 +443,10 @@ int bsz_plan_read_int(bsz_plan* plan, int inst, const char* name, int64_t* out) 
 
 int bsz_plan_mismatches(bsz_plan* plan, int kernel, int64_t* out) {
     return guard(plan->ctx, [&] { *out = plan->plan->mismatches(kernel); });
+}
+
+int bsz_plan_unproven(bsz_plan* plan, int kernel, int64_t* out) {
+    return guard(plan->ctx, [&] { *out = plan->plan->unproven(kernel); });
 }
 
 int bsz_plan_set_executor(bsz_plan* plan, int executor) {

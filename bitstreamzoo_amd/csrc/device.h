@@ -87,18 +87,29 @@ void launch_pointwise(hipStream_t s, PwOp op, uint64_t* dst, const uint64_t* a, 
 void launch_fill(hipStream_t s, uint64_t* dst, bool ones, const Geom& g);
 void launch_shl(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t k, const Geom& g);
 void launch_shr(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t k, const Geom& g);
-// Single-pass stream addition: wave ballot carry scan + decoupled lookback
-// across tiles.  `scan_state` holds AddScanState::bytes(g) zero-initialised
-// bytes; the kernel leaves it reusable for the next launch.
+// Stream addition: a main launch (one wave per segment of kRows rows, each
+// settling its own carry-in from at most kWarm rows below it) and a fix-up
+// launch that does nothing unless some segment's rows below all propagated.
+// `scan_state` holds AddScanState::bytes(g) zero-initialised bytes (a count,
+// a carry per segment, a bitmap of unsettled segments); the two launches
+// leave it reusable for the next addition of the same geometry.
 struct AddScanState {
-    static constexpr int kTileWords = 1024;  // 4 waves x 4 rows x 64 words
-    static uint64_t tiles(const Geom& g) { return (g.alloc_words() + kTileWords - 1) / kTileWords; }
-    static size_t bytes(const Geom& g) { return 64 + tiles(g) * 4; }
+    static constexpr int kRows = 4;
+    static constexpr int kWarm = 4;
+    static uint64_t segments(const Geom& g) { return (g.rows + kRows - 1) / kRows; }
+    static size_t bytes(const Geom& g) { return 64 + segments(g) * 4 + (segments(g) + 31) / 32 * 4; }
 };
 void launch_add(hipStream_t s, uint64_t* dst, const uint64_t* a, const uint64_t* b, const Geom& g, void* scan_state);
-// popcount / is_nonzero bounded to the bitlength, accumulated into *slot
-// (atomic add / atomic or); the caller zeroes the slot first.
-void launch_popcount(hipStream_t s, unsigned long long* slot, const uint64_t* src, const Geom& g);
+// A popcount accumulator: kPopSlots u64 slots kPopSlotStride words apart
+// (kPopSlotWords words in all, zeroed by the caller before the launch).
+constexpr int kPopSlots = 64;
+constexpr int kPopSlotStride = 16;
+constexpr size_t kPopSlotWords = (size_t)kPopSlots * kPopSlotStride;
+// popcount bounded to the bitlength, added into the accumulator `slots`.
+void launch_popcount(hipStream_t s, unsigned long long* slots, const uint64_t* src, const Geom& g);
+// The accumulator's total (synchronous).
+uint64_t read_popcount_slots(hipStream_t s, const unsigned long long* slots);
+// is_nonzero bounded to the bitlength, or-ed into *slot (zeroed by the caller).
 void launch_nonzero(hipStream_t s, unsigned long long* slot, const uint64_t* src, const Geom& g);
 void launch_zero_u64(hipStream_t s, unsigned long long* p, uint64_t n);
 // Load `n` host-side words into a row-padded buffer, truncated to the stream

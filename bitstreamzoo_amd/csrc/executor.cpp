@@ -391,9 +391,9 @@ public:
 
     PopRef popcount(const SV& a) override {
         auto p = std::make_shared<PerOpPop>();
-        p->dev = next_slot();
+        p->dev = next_slots(kPopSlotWords);
         p->s = s_;
-        launch_zero_u64(s_, p->dev, 1);
+        launch_zero_u64(s_, p->dev, kPopSlotWords);
         launch_popcount(s_, p->dev, a->buf->ptr, g_);
         return p;
     }
@@ -401,17 +401,14 @@ public:
     int64_t resolve(const PopRef& pr) override {
         if (!pr->ready) {
             auto* p = static_cast<PerOpPop*>(pr.get());
-            unsigned long long v = 0;
-            BSZ_HIP(hipMemcpyAsync(&v, p->dev, 8, hipMemcpyDeviceToHost, s_));
-            BSZ_HIP(hipStreamSynchronize(s_));
-            p->value = (int64_t)v;
+            p->value = (int64_t)read_popcount_slots(s_, p->dev);
             p->ready = true;
         }
         return pr->value;
     }
 
     bool is_nonzero(const SV& a) override {
-        unsigned long long* slot = next_slot();
+        unsigned long long* slot = next_slots(1);
         launch_zero_u64(s_, slot, 1);
         launch_nonzero(s_, slot, a->buf->ptr, g_);
         unsigned long long v = 0;
@@ -442,7 +439,7 @@ private:
     size_t slot_used_ = 0;
     SV zero_, ones_;
     int64_t launches0_ = 0;
-    static constexpr size_t kSlotsPerBlock = 4096;
+    static constexpr size_t kSlotsPerBlock = 1 << 16;
 
     SV fresh() {
         auto v = std::make_shared<StreamVal>();
@@ -450,14 +447,17 @@ private:
         return v;
     }
 
-    unsigned long long* next_slot() {
-        if (slot_blocks_.empty() || slot_used_ == kSlotsPerBlock) {
+    // n consecutive device words for a popcount accumulator or a flag
+    unsigned long long* next_slots(size_t n) {
+        if (slot_blocks_.empty() || slot_used_ + n > kSlotsPerBlock) {
             void* p = nullptr;
             BSZ_HIP(hipMalloc(&p, kSlotsPerBlock * 8));
             slot_blocks_.push_back(p);
             slot_used_ = 0;
         }
-        return reinterpret_cast<unsigned long long*>(slot_blocks_.back()) + slot_used_++;
+        unsigned long long* r = reinterpret_cast<unsigned long long*>(slot_blocks_.back()) + slot_used_;
+        slot_used_ += n;
+        return r;
     }
 };
 

@@ -310,8 +310,26 @@ public:
         return out;
     }
 
-    ModuleP load(const PlanSpec& p, double* compile_ms) {
+    // Compiles every kernel of `p`; a kernel whose code object overflows the
+    // architectural VGPRs is regenerated smaller (shrink_kernel) and compiled
+    // again, and the plan is refused if one still overflows.
+    std::vector<KernelRegs> compile_clean(PlanSpec& p, double* compile_ms) {
         compile_all(p, compile_ms);
+        std::vector<KernelRegs> regs = regs_of(p);
+        for (int round = 0; round < 12; ++round) {
+            bool changed = false;
+            for (size_t k = 0; k < p.kernels.size(); ++k)
+                if (!regs_clean(regs[k]) && shrink_kernel(p, k)) changed = true;
+            if (!changed) break;
+            compile_all(p, compile_ms);
+            regs = regs_of(p);
+        }
+        refuse_unclean(p, regs);
+        return regs;
+    }
+
+    ModuleP load(PlanSpec& p, double* compile_ms) {
+        compile_clean(p, compile_ms);
         const std::string arch = default_arch();
         auto m = std::make_shared<Module>();
         for (const KernelSpec& k : p.kernels) {
@@ -494,6 +512,13 @@ public:
         off_inst_ = a.take(sizeof(InstDesc) * n_inst);
         off_chunk_inst_ = n_inst > 1 ? a.take(4ull * n_chunks) : 0;
         off_ctl_ = a.take(4ull * (4 + 4 * K));
+        // unproven-chunk bitmaps of the stateful kernels, one region (cleared
+        // with the ctl block whenever the layout is set)
+        unp_words_ = (n_chunks + 31) / 32;
+        size_t n_stateful = 0;
+        for (size_t k = 0; k < K; ++k) n_stateful += P.kernels[k].n_state ? 1 : 0;
+        off_unp_ = a.take(4ull * std::max<size_t>(1, unp_words_ * n_stateful));
+        unp_bytes_ = 4ull * std::max<size_t>(1, unp_words_ * n_stateful);
         for (size_t k = 0; k < K; ++k) total_pop += P.kernels[k].pops.size() * n_inst;
         uint32_t max_chunks = 1;
         for (const InstDesc& d : inst_h_) max_chunks = std::max(max_chunks, d.n_chunks);
@@ -501,7 +526,7 @@ public:
         while (spread_log_ < (uint32_t)kMaxPopSpreadLog && (4u << spread_log_) < max_chunks) ++spread_log_;
         pop_stride_ = total_pop << spread_log_;
         off_pop_ = a.take(8ull * std::max<size_t>(1, 2 * pop_stride_));
-        size_t pop_cursor = 0;
+        size_t pop_cursor = 0, unp_cursor = 0;
         for (size_t k = 0; k < K; ++k) {
             const KernelSpec& ks = P.kernels[k];
             KRes& r = kres_[k];
@@ -514,6 +539,7 @@ public:
             r.pop = off_pop_ + (8ull << spread_log_) * pop_cursor;
             pop_cursor += (size_t)r.npop * n_inst;
             if (r.nst) {
+                r.unproven = off_unp_ + 4ull * unp_words_ * unp_cursor++;
                 r.st_u = a.take(8ull * n_chunks * r.nst);
                 r.st_t = a.take(8ull * n_chunks * r.nst);
                 r.partial = a.take(4ull * n_chunks * std::max(1, r.npop));
@@ -529,6 +555,7 @@ public:
         } else {
             // epoch 0 and both popcount banks clear for the new layout
             BSZ_HIP(hipMemsetAsync(ptr(off_ctl_), 0, 4ull * (4 + 4 * K), s));
+            BSZ_HIP(hipMemsetAsync(ptr(off_unp_), 0, unp_bytes_, s));
             BSZ_HIP(hipMemsetAsync(ptr(off_pop_), 0, 8ull * std::max<size_t>(1, 2 * pop_stride_), s));
         }
         in_tab_h_.assign(K, {});
@@ -584,6 +611,7 @@ public:
             A.partial = r.nst ? (unsigned*)ptr(r.partial) : nullptr;
             A.ctl = (unsigned*)ptr(off_ctl_);
             A.kstat = (unsigned*)ptr(off_ctl_ + 4ull * (4 + 4 * k));
+            A.unproven = r.nst ? (unsigned*)ptr(r.unproven) : nullptr;
             A.pop_stride = pop_stride_;
             A.n_chunks = n_chunks;
             A.n_inst = n_inst;
@@ -657,13 +685,17 @@ public:
         }
         throw RunError("popcount not in plan");
     }
-    // mismatching chunks seen by kernel k in the last run (sync)
-    uint32_t mismatches(hipStream_t s, size_t k) {
+    // stat `which` of kernel k after the last run (sync): 0 chunks whose
+    // speculative start state was wrong, 3 chunks whose warm-up did not prove
+    // their start state exact
+    uint32_t kernel_stat(hipStream_t s, size_t k, int which) {
+        if (k >= kres_.size()) throw RunError("no kernel " + std::to_string(k) + " in this plan");
         uint32_t v = 0;
-        BSZ_HIP(hipMemcpyAsync(&v, ptr(off_ctl_ + 4ull * (4 + 4 * k)), 4, hipMemcpyDeviceToHost, s));
+        BSZ_HIP(hipMemcpyAsync(&v, ptr(off_ctl_ + 4ull * (4 + 4 * k) + 4ull * which), 4, hipMemcpyDeviceToHost, s));
         BSZ_HIP(hipStreamSynchronize(s));
         return v;
     }
+    uint32_t mismatches(hipStream_t s, size_t k) { return kernel_stat(s, k, 0); }
     int64_t launches() const { return launches_; }
     size_t device_bytes() const { return dev_bytes_; }
 
@@ -671,11 +703,12 @@ private:
     struct KRes {
         size_t nin = 0, nout = 0;
         int npop = 0, nst = 0;
-        size_t in_tab = 0, out_tab = 0, pop = 0, st_u = 0, st_t = 0, partial = 0;
+        size_t in_tab = 0, out_tab = 0, pop = 0, st_u = 0, st_t = 0, partial = 0, unproven = 0;
     };
     void* dev_ = nullptr;
     size_t dev_bytes_ = 0;
     size_t off_inst_ = 0, off_chunk_inst_ = 0, off_ctl_ = 0, off_pop_ = 0, total_pop_ = 0, pop_stride_ = 0;
+    size_t off_unp_ = 0, unp_words_ = 0, unp_bytes_ = 0;
     uint32_t spread_log_ = 0;
     std::vector<KRes> kres_;
     std::vector<InstDesc> inst_h_;
@@ -1173,10 +1206,7 @@ std::unique_ptr<FusedPlan> FusedPlan::record(hipStream_t s, const Program& prog,
                                  std::max(1, opt.compile_shards));
         // a shard job (one of several processes filling the cache) does not
         // report registers: its plan's other kernels may not exist yet
-        if (std::max(1, opt.compile_shards) == 1) {
-            I.compiled_regs = I.rec->jit().regs_of(*I.spec);
-            refuse_unclean(*I.spec, I.compiled_regs);
-        }
+        if (std::max(1, opt.compile_shards) == 1) I.compiled_regs = I.rec->jit().compile_clean(*I.spec, &I.rec->compile_ms());
     }
     return fp;
 }
@@ -1277,7 +1307,7 @@ void FusedPlan::set_executor(int executor) {
             const size_t st = AddScanState::bytes(g);
             BSZ_HIP(hipMalloc(&P.add_state, st));
             BSZ_HIP(hipMemsetAsync(P.add_state, 0, st, I.s));
-            BSZ_HIP(hipMalloc(&P.pops, 8 * std::max<size_t>(1, I.spec->g.pops.size())));
+            BSZ_HIP(hipMalloc(&P.pops, 8 * kPopSlotWords * std::max<size_t>(1, I.spec->g.pops.size())));
         }
         BSZ_HIP(hipStreamSynchronize(I.s));
     }
@@ -1319,8 +1349,9 @@ void FusedPlan::run() {
             }
         }
         if (!g.pops.empty()) {
-            launch_zero_u64(I.s, P.pops, g.pops.size());
-            for (size_t p = 0; p < g.pops.size(); ++p) launch_popcount(I.s, P.pops + p, ptr_of(g.pops[p]), geo);
+            launch_zero_u64(I.s, P.pops, g.pops.size() * kPopSlotWords);
+            for (size_t p = 0; p < g.pops.size(); ++p)
+                launch_popcount(I.s, P.pops + p * kPopSlotWords, ptr_of(g.pops[p]), geo);
         }
     }
 }
@@ -1331,7 +1362,9 @@ void FusedPlan::capture() {
     if (I.executor == 0) {
         if (graph_) { (void)hipGraphExecDestroy((hipGraphExec_t)graph_); graph_ = nullptr; }
         const int64_t pops = I.spec->g.pops.empty() ? 0 : 1 + (int64_t)I.spec->g.pops.size();
-        const int64_t per_inst = (int64_t)I.perop_order.size() + pops;
+        int64_t adds = 0;  // an addition is a main and a fix-up launch
+        for (int32_t n : I.perop_order) adds += I.spec->g.nodes[n].op == NOp::Add ? 1 : 0;
+        const int64_t per_inst = (int64_t)I.perop_order.size() + adds + pops;
         graph_ = capture_graph(I.s, per_inst * (int64_t)I.geoms.size(), [&] { run(); });
         return;
     }
@@ -1356,9 +1389,7 @@ int64_t FusedPlan::read_int(int inst, const std::string& name, bool refresh) {
         if (io.name != name) continue;
         if (io.pop < 0) return io.value;
         if (I.executor == 0) {
-            unsigned long long v = 0;
-            BSZ_HIP(hipMemcpyAsync(&v, I.perop.at(inst).pops + io.pop, 8, hipMemcpyDeviceToHost, I.s));
-            BSZ_HIP(hipStreamSynchronize(I.s));
+            const uint64_t v = read_popcount_slots(I.s, I.perop.at(inst).pops + (size_t)io.pop * kPopSlotWords);
             return I.geoms.at(inst).bitlength <= 0 ? 0 : (int64_t)v;
         }
         if (refresh || I.pops_host.empty()) I.rt->read_pops(I.s, I.pops_host);
@@ -1369,6 +1400,7 @@ int64_t FusedPlan::read_int(int inst, const std::string& name, bool refresh) {
 }
 
 uint32_t FusedPlan::mismatches(int kernel) { return impl_->rt->mismatches(impl_->s, kernel); }
+uint32_t FusedPlan::unproven(int kernel) { return impl_->rt->kernel_stat(impl_->s, kernel, 3); }
 
 std::string FusedPlan::describe_json(const std::string& options_array) const {
     const Json arr = json_parse(options_array);

@@ -62,6 +62,7 @@ public:
     void replay();     // launch the captured graph
     int64_t read_int(int inst, const std::string& name, bool refresh);
     uint32_t mismatches(int kernel);
+    uint32_t unproven(int kernel);   // chunks whose warm-up left a state unproven in the last run
     std::string info_json() const;
     // Kernel shapes of the recorded program under each schedule of a JSON
     // array of option objects (partitioning only, no code generation).

@@ -43,7 +43,7 @@ struct InstDesc { u64 rows; u64 lw; u64 lastmask; u32 chunk_begin; u32 n_chunks;
 struct KArgs {
   const u64* const* in; u64* const* out; u64* pop_acc;
   const InstDesc* inst; const u32* chunk_inst;
-  u64* st_u; u64* st_t; u32* partial; u32* ctl; u32* kstat;
+  u64* st_u; u64* st_t; u32* partial; u32* ctl; u32* kstat; u32* unproven;
   u64 pop_stride; u32 n_chunks; u32 n_inst; u32 mode; u32 last; u32 spread_log; u32 pad_;
 };
 #define AGENT __HIP_MEMORY_SCOPE_AGENT
@@ -394,8 +394,12 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     }
     s << "  u32 c = 0;\n";
     if (has_state) {
-        // mode 1 (one workgroup): mark every chunk whose speculative start
-        // state differs from its predecessor's end state, in parallel
+        // mode 1 (one workgroup).  Chunks whose warm-up proved their start
+        // state exact need nothing; when the main launch left no chunk
+        // unproven the launch ends at once.  Otherwise every unproven chunk
+        // whose speculative start state differs from its predecessor's end
+        // state is marked, in parallel, and one wave recomputes the marked
+        // chunks in order.
         // dynamic LDS: only the fix-up launch asks for it, so the main launch's
         // occupancy is not limited by the bitmap
         s << "  extern __shared__ u32 s_dyn[];\n";
@@ -406,12 +410,21 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "  u32 last_c = 0xfffffffeu;\n";
         for (auto& kv : state_of) s << "  u64 st" << kv.second << " = 0ull;\n";
         s << "  if (fix) {\n";
+        s << "    const u32 n_unproven = aload32(&A.kstat[2]);\n";
+        s << "    if (n_unproven == 0u) {\n";
+        s << "      if (threadIdx.x == 0) {\n";
+        s << "        astore32(&A.kstat[0], 0u);\n        astore32(&A.kstat[1], 0u);\n        astore32(&A.kstat[3], 0u);\n";
+        s << "        if (A.last) astore32(&A.ctl[0], aload32(&A.ctl[0]) + 1u);\n";
+        s << "      }\n";
+        s << "      return;\n";
+        s << "    }\n";
         s << "    const bool fits = A.n_chunks <= 32u * " << kFixBitmapWords << "u;\n";
         s << "    const u32 nbw = fits ? (A.n_chunks + 31u) / 32u : 0u;\n";
         s << "    for (u32 i = threadIdx.x; i < nbw; i += blockDim.x) s_bits[i] = 0u;\n";
         s << "    if (threadIdx.x == 0) s_any = 0u;\n";
         s << "    __syncthreads();\n";
         s << "    for (u32 x = 1u + threadIdx.x; fits && x < A.n_chunks; x += blockDim.x) {\n";
+        s << "      if (!((aload32(&A.unproven[x >> 5]) >> (x & 31u)) & 1u)) continue;\n";
         s << "      const u32 xi = A.chunk_inst ? A.chunk_inst[x] : 0u;\n";
         s << "      if (x == A.inst[xi].chunk_begin) continue;\n";
         s << "      bool bad = false;\n";
@@ -420,6 +433,9 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "      if (bad) { atomicOr(&s_bits[x >> 5], 1u << (x & 31u)); atomicAdd(&s_any, 1u); }\n";
         s << "    }\n";
         s << "    __syncthreads();\n";
+        // the unproven marks and their count start from zero in the next run
+        s << "    for (u32 i = threadIdx.x; i < (A.n_chunks + 31u) / 32u; i += blockDim.x) astore32(&A.unproven[i], 0u);\n";
+        s << "    if (threadIdx.x == 0) { astore32(&A.kstat[2], 0u); astore32(&A.kstat[3], n_unproven); }\n";
         s << "    if (threadIdx.x >= 64u) return;\n";
         s << "    n_mis = s_any;\n";
         s << "    use_bits = fits;\n";
@@ -484,89 +500,69 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     for (int j = 0; j < NOUT; ++j) s << "    u64* __restrict__ out" << j << " = tab_out(A.out, (u64)ii * " << NOUT << "u + " << j << "u);\n";
     s << "    const u64 r_begin = (u64)cc * " << CR << "ull;\n";
     s << "    const u64 r_end = (r_begin + " << CR << "ull < D.rows) ? r_begin + " << CR << "ull : D.rows;\n";
-    if (has_state) {
-        s << "    const u64 warm = (!fix && cc > 0) ? (r_begin < " << WR << "ull ? r_begin : " << WR << "ull) : 0ull;\n";
-        s << "    if (!fix) {\n";
-        for (auto& kv : state_of) s << "      st" << kv.second << " = 0ull;\n";
-        s << "    } else if (c != last_c + 1u) {\n";
-        for (auto& kv : state_of)
-            s << "      st" << kv.second << " = aload64(&A.st_t[(u64)(c - 1u) * " << NS << " + " << kv.second << "u]);\n";
-        s << "    }\n";
-        // the start state a recomputed chunk used
-        s << "    if (fix) {\n";
-        for (auto& kv : state_of)
-            s << "      if (lane == " << (kv.second % 64) << ") astore64(&A.st_u[(u64)c * " << NS << " + " << kv.second
-              << "u], st" << kv.second << ");\n";
-        s << "    }\n";
-    } else {
-        s << "    const u64 warm = 0ull;\n";
-    }
-    for (int p = 0; p < NPOP; ++p) s << "    u32 pc" << p << " = 0u;\n";
+
     // Interleaved rows (kernels without state only): chunk cc of an instance
     // with n chunks takes rows cc, cc + n, cc + 2n, ..., so the waves running
     // at one time read one contiguous window of every stream rather than one
     // distant segment each.
     const bool inter = opt.interleave && !has_state;
     ks.interleave = inter;
-    if (inter) {
-        s << "    const u64 istride = (u64)D.n_chunks;\n";
-        s << "    for (u64 r = cc; r < D.rows; r += istride * " << U << "ull) {\n";
-    } else {
-        s << "    for (u64 r = r_begin - warm; r < r_end; r += " << U << "ull) {\n";
-    }
-    // load phase
-    for (int u = 0; u < U; ++u) {
-        // The row index goes through readfirstlane so that addresses are a
-        // scalar row base plus the lane offset (no per-lane pointer per stream).
-        const std::string rexpr = inter ? "r + " + std::to_string(u) + "ull * istride" : "r + " + std::to_string(u) + "ull";
-        s << "      const u64 ru" << u << " = (u64)uni((u32)(" << rexpr << "));\n";
-        s << "      const long long w" << u << " = (long long)(ru" << u << " * 64ull + lane);\n";
-        if (U > 1) s << "      const bool ok" << u << " = " << rexpr << " < " << (inter ? "D.rows" : "r_end") << ";\n";
-        else s << "      const bool ok" << u << " = true;\n";
-        // row base (wave-uniform, scalar) + lane: loads take the SGPR-base +
-        // VGPR-offset form, so no per-lane pointer is kept per stream
-        for (int32_t a : plain_inputs) {
-            const std::string rb = "urow(in" + std::to_string(input_index.at(a)) + " + ru" + std::to_string(u) + " * 64ull)";
-            if (U > 1)
-                s << "      const u64 x" << a << "_" << u << " = ok" << u << " ? lane_ld(" << rb << ", off8) : 0ull;\n";
-            else
-                s << "      const u64 x" << a << "_" << u << " = lane_ld(" << rb << ", off8);\n";
+    std::map<int32_t, std::vector<int>> out_of, pop_of;
+    for (int j = 0; j < NOUT; ++j) out_of[ks.outputs[j]].push_back(j);
+    for (int p = 0; p < NPOP; ++p) pop_of[g.pops[ks.pops[p]]].push_back(p);
+
+    // Loads of U rows starting at row r (row u: `r + u`, or `r + u * istride`
+    // when interleaved); row u exists while its index is below `limit`.
+    auto emit_loads = [&](int UU, const std::string& limit) {
+        for (int u = 0; u < UU; ++u) {
+            // The row index goes through readfirstlane so that addresses are a
+            // scalar row base plus the lane offset (no per-lane pointer per stream).
+            const std::string rexpr = inter ? "r + " + std::to_string(u) + "ull * istride" : "r + " + std::to_string(u) + "ull";
+            s << "      const u64 ru" << u << " = (u64)uni((u32)(" << rexpr << "));\n";
+            s << "      const long long w" << u << " = (long long)(ru" << u << " * 64ull + lane);\n";
+            if (UU > 1) s << "      const bool ok" << u << " = " << rexpr << " < " << limit << ";\n";
+            else s << "      const bool ok" << u << " = true;\n";
+            for (int32_t a : plain_inputs) {
+                const std::string rb = "urow(in" + std::to_string(input_index.at(a)) + " + ru" + std::to_string(u) + " * 64ull)";
+                if (UU > 1)
+                    s << "      const u64 x" << a << "_" << u << " = ok" << u << " ? lane_ld(" << rb << ", off8) : 0ull;\n";
+                else
+                    s << "      const u64 x" << a << "_" << u << " = lane_ld(" << rb << ", off8);\n";
+            }
+            for (auto& ow : off_words) {
+                const std::string d = ow.second < 0 ? " - " + std::to_string(-ow.second) + "ll" : " + " + std::to_string(ow.second) + "ll";
+                s << "      const u64 z" << ow.first << "_" << (ow.second < 0 ? "m" : "p") << std::llabs(ow.second) << "_" << u
+                  << " = ok" << u << " ? ldr(urow(in" << input_index.at(ow.first) << " + ru" << u << " * 64ull), (long long)lane"
+                  << d << ", w" << u << d << ", NW) : 0ull;\n";
+            }
+            for (int32_t n : ext_shifts) {
+                const GNode& x = g.nodes[n];
+                const long long q = (long long)(x.k / 64);
+                const unsigned b = (unsigned)(x.k % 64);
+                const GNode& src = g.nodes[x.a];
+                auto word = [&](long long d) -> std::string {
+                    if (src.op == NOp::Zero) return "0ull";
+                    std::string wd = "w" + std::to_string(u) + (d < 0 ? " - " + std::to_string(-d) + "ll" : " + " + std::to_string(d) + "ll");
+                    if (src.op == NOp::Ones) return "word_mask(" + wd + ", D.lw, D.lastmask)";
+                    if (d == 0) return "x" + std::to_string(x.a) + "_" + std::to_string(u);
+                    return "z" + std::to_string(x.a) + "_" + (d < 0 ? "m" : "p") + std::to_string(std::llabs(d)) + "_" + std::to_string(u);
+                };
+                std::string e = x.op == NOp::Shl
+                    ? "fsl(" + word(-q) + ", " + (b ? word(-q - 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)"
+                    : "fsr(" + word(q) + ", " + (b ? word(q + 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)";
+                s << "      const u64 y" << n << "_" << u << " = ok" << u << " ? " << e << " : 0ull;\n";
+            }
         }
-        for (auto& ow : off_words) {
-            const std::string d = ow.second < 0 ? " - " + std::to_string(-ow.second) + "ll" : " + " + std::to_string(ow.second) + "ll";
-            s << "      const u64 z" << ow.first << "_" << (ow.second < 0 ? "m" : "p") << std::llabs(ow.second) << "_" << u
-              << " = ok" << u << " ? ldr(urow(in" << input_index.at(ow.first) << " + ru" << u << " * 64ull), (long long)lane"
-              << d << ", w" << u << d << ", NW) : 0ull;\n";
-        }
-        for (int32_t n : ext_shifts) {
-            const GNode& x = g.nodes[n];
-            const long long q = (long long)(x.k / 64);
-            const unsigned b = (unsigned)(x.k % 64);
-            const GNode& src = g.nodes[x.a];
-            auto word = [&](long long d) -> std::string {
-                if (src.op == NOp::Zero) return "0ull";
-                std::string wd = "w" + std::to_string(u) + (d < 0 ? " - " + std::to_string(-d) + "ll" : " + " + std::to_string(d) + "ll");
-                if (src.op == NOp::Ones) return "word_mask(" + wd + ", D.lw, D.lastmask)";
-                if (d == 0) return "x" + std::to_string(x.a) + "_" + std::to_string(u);
-                return "z" + std::to_string(x.a) + "_" + (d < 0 ? "m" : "p") + std::to_string(std::llabs(d)) + "_" + std::to_string(u);
-            };
-            std::string e = x.op == NOp::Shl
-                ? "fsl(" + word(-q) + ", " + (b ? word(-q - 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)"
-                : "fsr(" + word(q) + ", " + (b ? word(q + 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)";
-            s << "      const u64 y" << n << "_" << u << " = ok" << u << " ? " << e << " : 0ull;\n";
-        }
-    }
-    // compute phase
-    for (int u = 0; u < U; ++u) {
+    };
+
+    // The nodes of row u.  warm = true: a warm-up row before the chunk; it
+    // stores nothing and tracks, per value, the lowest lane from which the
+    // value is certainly exact (L, 0..64) and, per state word, whether it is
+    // exact (xs): an inexact start state damages at most the low lanes of a
+    // row, and an advance's state (the operand's lane 63) or a carry out of
+    // a lane that neither propagates nor lies in the damaged range is exact.
+    auto emit_compute = [&](int u, bool warm) {
         s << "      if (ok" << u << ") {\n";
-        if (has_state) {
-            // the speculative state this chunk reached at its first row (U[c])
-            s << "        if (r + " << u << "ull == r_begin && !fix) {\n";
-            for (auto& kv : state_of)
-                s << "          if (lane == " << (kv.second % 64) << ") A.st_u[(u64)c * " << NS << " + " << kv.second
-                  << "u] = st" << kv.second << ";\n";
-            s << "        }\n";
-        }
         s << "        const u64 m = word_mask(w" << u << ", D.lw, D.lastmask);\n";
         auto ref = [&](int32_t a) -> std::string {
             const GNode& x = g.nodes[a];
@@ -575,30 +571,31 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
             if (x.op == NOp::Ones) return "m";
             return "x" + std::to_string(a) + "_" + std::to_string(u);
         };
+        auto lref = [&](int32_t a) -> std::string { return in_kernel.count(a) ? "L" + std::to_string(a) : "0u"; };
+        auto lmax = [&](const std::string& x, const std::string& y) -> std::string {
+            if (x == "0u") return y;
+            if (y == "0u" || x == y) return x;
+            return "(" + x + " > " + y + " ? " + x + " : " + y + ")";
+        };
         // outputs and popcounts of each node are emitted right after it, so
         // its register is free again at once
-        std::map<int32_t, std::vector<int>> out_of, pop_of;
-        for (int j = 0; j < NOUT; ++j) out_of[ks.outputs[j]].push_back(j);
-        for (int p = 0; p < NPOP; ++p) pop_of[g.pops[ks.pops[p]]].push_back(p);
-        const bool guard = has_state;
         auto sink = [&](int32_t n) {
+            if (warm) return;
             auto oi = out_of.find(n);
             auto pi = pop_of.find(n);
             if (oi == out_of.end() && pi == pop_of.end()) return;
-            if (guard) s << "        if (r + " << u << "ull >= r_begin) {\n";
             if (oi != out_of.end())
                 for (int j : oi->second)
-                    s << "          lane_st(urow_w(out" << j << " + ru" << u << " * 64ull), off8, " << ref(n) << " & m);\n";
+                    s << "        lane_st(urow_w(out" << j << " + ru" << u << " * 64ull), off8, " << ref(n) << " & m);\n";
             if (pi != pop_of.end())
-                for (int p : pi->second) s << "          pc" << p << " += (u32)__builtin_popcountll(" << ref(n) << " & m);\n";
-            if (guard) s << "        }\n";
+                for (int p : pi->second) s << "        pc" << p << " += (u32)__builtin_popcountll(" << ref(n) << " & m);\n";
             // without it the scheduler may sink every store to the end of the
             // step and keep all the outputs of a row live at once
             if (opt.store_order) s << "        __builtin_amdgcn_sched_barrier(0);\n";
         };
         for (int32_t n : ks.nodes) {
             const GNode& x = g.nodes[n];
-            const std::string v = "v" + std::to_string(n);
+            const std::string v = "v" + std::to_string(n), L = "L" + std::to_string(n);
             switch (x.op) {
                 case NOp::Zero: s << "        const u64 " << v << " = 0ull;\n"; break;
                 case NOp::Ones: s << "        const u64 " << v << " = m;\n"; break;
@@ -616,6 +613,18 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                     s << "        const u64 " << v << " = s" << t << " + (((u" << t << " ^ G" << t << " ^ P" << t
                       << ") >> lane) & 1ull);\n";
                     s << "        " << st << " = (u64)((t" << t << " < G" << t << ") | (u" << t << " < t" << t << "));\n";
+                    if (warm) {
+                        // lanes that do not propagate a carry (they generate
+                        // or kill it) at or above the damaged range end it
+                        const std::string xs = "xs" + std::to_string(state_of.at(n));
+                        const std::string li = lmax(lref(x.a), lref(x.b));
+                        s << "        const u64 np" << t << " = (~P" << t << " | G" << t << ") & (" << li
+                          << " >= 64u ? 0ull : (~0ull << " << li << "));\n";
+                        s << "        const bool ex" << t << " = " << xs << " != 0u && " << li << " == 0u;\n";
+                        s << "        const u32 " << L << " = ex" << t << " ? 0u : (np" << t << " ? (u32)__builtin_ctzll(np"
+                          << t << ") + 1u : 64u);\n";
+                        s << "        " << xs << " = (ex" << t << " || np" << t << " != 0ull) ? 1u : 0u;\n";
+                    }
                     break;
                 }
                 case NOp::Shl: case NOp::Shr: {
@@ -629,15 +638,36 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                         s << "        const u64 " << v << " = fsl(" << ref(x.a) << ", prev_word(" << ref(x.a) << ", " << st
                           << "), " << x.k << "u);\n";
                         s << "        " << st << " = lane63(" << ref(x.a) << ");\n";
+                        if (warm) {
+                            // lane 0 takes the state; lane i > 0 takes operand lanes i and i-1
+                            const std::string xs = "xs" + std::to_string(state_of.at(n)), la = lref(x.a);
+                            s << "        const u32 " << L << " = " << la << " == 0u ? (" << xs << " ? 0u : 1u) : (" << la
+                              << " < 64u ? " << la << " + 1u : 64u);\n";
+                            s << "        " << xs << " = " << la << " <= 63u ? 1u : 0u;\n";
+                        }
                     }
                     break;
                 }
                 case NOp::Input: break;
             }
+            if (warm) {
+                switch (x.op) {
+                    case NOp::Zero: case NOp::Ones: s << "        const u32 " << L << " = 0u;\n"; break;
+                    case NOp::Not: s << "        const u32 " << L << " = " << lref(x.a) << ";\n"; break;
+                    case NOp::And: case NOp::Or: case NOp::Xor:
+                        s << "        const u32 " << L << " = " << lmax(lref(x.a), lref(x.b)) << ";\n";
+                        break;
+                    case NOp::Shl: case NOp::Shr:
+                        if (!in_kernel.count(x.a)) s << "        const u32 " << L << " = 0u;\n";
+                        else if (x.k == 0) s << "        const u32 " << L << " = " << lref(x.a) << ";\n";
+                        break;
+                    default: break;
+                }
+            }
             sink(n);
         }
         // outputs / popcounts of values this kernel does not compute (inputs, constants)
-        {
+        if (!warm) {
             std::set<int32_t> done(ks.nodes.begin(), ks.nodes.end());
             std::set<int32_t> rest;
             for (auto& kv : out_of) if (!done.count(kv.first)) rest.insert(kv.first);
@@ -645,7 +675,55 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
             for (int32_t n : rest) sink(n);
         }
         s << "      }\n";
+    };
+
+    if (has_state) {
+        s << "    const u64 warm = (!fix && cc > 0) ? (r_begin < " << WR << "ull ? r_begin : " << WR << "ull) : 0ull;\n";
+        s << "    if (!fix) {\n";
+        for (auto& kv : state_of) s << "      st" << kv.second << " = 0ull;\n";
+        s << "    } else if (c != last_c + 1u) {\n";
+        for (auto& kv : state_of)
+            s << "      st" << kv.second << " = aload64(&A.st_t[(u64)(c - 1u) * " << NS << " + " << kv.second << "u]);\n";
+        s << "    }\n";
+        // Warm-up (main launch, chunks after an instance's first): the rows
+        // before the chunk, from a zero state, storing nothing.  A chunk
+        // whose warm-up proves every state exact is final; the others are
+        // marked for the fix-up launch.
+        s << "    if (warm) {\n";
+        for (auto& kv : state_of) s << "      u32 xs" << kv.second << " = 0u;\n";
+        s << "      for (u64 r = r_begin - warm; r < r_begin; r += " << U << "ull) {\n";
+        emit_loads(U, "r_begin");
+        for (int u = 0; u < U; ++u) emit_compute(u, true);
+        s << "      }\n";
+        s << "      u32 proven = 1u;\n";
+        for (auto& kv : state_of) s << "      proven &= xs" << kv.second << ";\n";
+        s << "      if (!proven && lane == 0) {\n";
+        s << "        __hip_atomic_fetch_or(&A.unproven[c >> 5], 1u << (c & 31u), __ATOMIC_RELAXED, AGENT);\n";
+        s << "        __hip_atomic_fetch_add(&A.kstat[2], 1u, __ATOMIC_RELAXED, AGENT);\n";
+        s << "      }\n";
+        s << "    }\n";
+        // the state this chunk starts from (U[c]): speculative in the main
+        // launch, the predecessor's true end state in a recomputation
+        s << "    if (fix) {\n";
+        for (auto& kv : state_of)
+            s << "      if (lane == " << (kv.second % 64) << ") astore64(&A.st_u[(u64)c * " << NS << " + " << kv.second
+              << "u], st" << kv.second << ");\n";
+        s << "    } else {\n";
+        for (auto& kv : state_of)
+            s << "      if (lane == " << (kv.second % 64) << ") A.st_u[(u64)c * " << NS << " + " << kv.second << "u] = st"
+              << kv.second << ";\n";
+        s << "    }\n";
     }
+    for (int p = 0; p < NPOP; ++p) s << "    u32 pc" << p << " = 0u;\n";
+    if (inter) {
+        s << "    const u64 istride = (u64)D.n_chunks;\n";
+        s << "    for (u64 r = cc; r < D.rows; r += istride * " << U << "ull) {\n";
+        emit_loads(U, "D.rows");
+    } else {
+        s << "    for (u64 r = r_begin; r < r_end; r += " << U << "ull) {\n";
+        emit_loads(U, "r_end");
+    }
+    for (int u = 0; u < U; ++u) emit_compute(u, false);
     s << "    }\n";  // row loop
 
     // popcounts: one atomic per chunk into slot c mod 2^spread_log
@@ -909,6 +987,33 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
     }
     P.module_source = mod.str();
     return P;
+}
+
+bool shrink_kernel(PlanSpec& P, size_t k) {
+    KernelSpec& ks = P.kernels.at(k);
+    PlanOptions o = P.opt;
+    const bool wide = ks.words_per_lane == 2;
+    const int U = wide ? ks.rows_per_step / 2 : ks.rows_per_step;  // rows (or row pairs) per step
+    // under automatic occupancy the kernel keeps (at most) the occupancy it
+    // had, so fewer rows never buy a smaller register budget instead
+    if (P.opt.occupancy <= 0) o.occupancy = ks.occupancy;
+    if (U > 1) {
+        o.rows_per_step = std::max(1, U / 2);
+        o.words_per_lane = ks.words_per_lane;
+    } else if (wide) {
+        o.rows_per_step = 1;
+        o.words_per_lane = 1;
+    } else if (P.opt.occupancy <= 0 && ks.occupancy > 2) {
+        o.rows_per_step = 1;
+        o.occupancy = ks.occupancy - 1;
+    } else {
+        return false;
+    }
+    gen_kernel(P.g, o, ks, true);
+    ks.unit = (P.opt.nontemporal > 0 ? "#define BSZ_NT " + std::to_string(std::min(2, P.opt.nontemporal)) + "\n"
+                                     : std::string()) +
+              std::string(kKernelPrelude) + ks.source;
+    return true;
 }
 
 uint64_t source_hash(const std::string& s) { return fnv1a(s); }

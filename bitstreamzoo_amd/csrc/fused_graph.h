@@ -16,11 +16,18 @@
 //     chunk other than the first starts `warmup_rows` rows early with a zero
 //     state (speculation), records the state it reaches at its first row (U)
 //     and the state at its end (T).  Chunk c was exact iff U[c] == T[c-1].
+//     During the warm-up the wave also tracks, per value, the lowest lane
+//     from which it is certainly exact, and per state word whether it is:
+//     a wrong start state damages only the low lanes of a row, an advance's
+//     state is its operand's lane 63, and a carry is settled by any lane that
+//     generates or kills it above the damage.  A chunk whose warm-up proves
+//     every state exact is final; others are marked in a bitmap.
 //     The main launch has no cross-wave dependency at all (no tickets, no
 //     waits, no flags): wave w of the grid owns chunk w.  A second launch of
-//     the same kernel (one workgroup, mode 1) compares U[c] with T[c-1] for
-//     every chunk in parallel, marks the mismatches in an LDS bitmap, and one
-//     wave recomputes them in order from the predecessor's true end state,
+//     the same kernel (one workgroup, mode 1) ends at once when no chunk was
+//     marked; otherwise it compares U[c] with T[c-1] for every marked chunk in
+//     parallel, marks the mismatches in an LDS bitmap, and one wave
+//     recomputes them in order from the predecessor's true end state,
 //     following a changed end state into the next chunk.  The result is exact
 //     for every input; speculation only decides how much work is redone;
 //   * an advance of a value computed by an earlier kernel (any k) and `>>` are
@@ -113,6 +120,11 @@ struct PlanSpec {
 // describe a candidate schedule.
 PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit = true);
 
+// Regenerates kernel k of P with a smaller register footprint and the same
+// nodes: half the rows per step, then 8-byte lanes, then (automatic occupancy
+// only) one wave fewer per SIMD.  False when nothing is left to shrink.
+bool shrink_kernel(PlanSpec& P, size_t k);
+
 // Argument block every generated kernel takes by value (layout shared with the
 // generated source, see kKernelPrelude in fused_codegen.cpp).
 struct InstDesc {
@@ -132,7 +144,8 @@ constexpr int kFixThreads = 256;
 
 // ctl block of a plan: [0] epoch (runs completed); per kernel k, at
 // 4 + 4k: [0] chunks whose speculation failed in the last run, [1] chunks
-// recomputed.
+// recomputed, [2] chunks the current main launch could not prove exact
+// (cleared by the fix-up launch), [3] that count for the last run.
 struct KArgs {
     const uint64_t* const* in;        // [inst][n_in]
     uint64_t* const* out;             // [inst][n_out]
@@ -144,6 +157,7 @@ struct KArgs {
     unsigned* partial;                // [n_chunks][n_pop_k] (stateful kernels)
     unsigned* ctl;                    // the plan's ctl block
     unsigned* kstat;                  // this kernel's stats in it
+    unsigned* unproven;               // [ceil(n_chunks/32)] chunks whose warm-up left a state unproven
     uint64_t pop_stride;              // words between the two banks
     uint32_t n_chunks;
     uint32_t n_inst;

@@ -16,6 +16,7 @@
 #include "device.h"
 
 #include <atomic>
+#include <vector>
 
 namespace bsz {
 
@@ -102,105 +103,130 @@ __global__ __launch_bounds__(256) void k_shr(uint64_t* __restrict__ dst, const u
     dst[w] = r;
 }
 
-// One-pass stream addition.  Block = 4 waves; wave v owns rows 4v..4v+3 of
-// the tile (row = 64 consecutive words, one per lane).  Status words of the
-// decoupled lookback are tagged with the launch epoch so they never need
-// clearing:  bits 4.. tag, bits 2-3 kind (1 aggregate, 2 inclusive prefix),
-// bits 0-1 the carry transfer function (f(0), f(1)) or the carry-out.
-// ctl[0] tile ticket, ctl[1] finished tiles, ctl[2] epoch; the last tile to
-// finish resets ctl[0..1] and advances the epoch.
+// Stream addition, one segment of kAddRows rows (64 words each) per wave.
+// Lane sums, a ballot of the lanes that generate a carry (G) and of those
+// that generate or propagate one (GP), and the carries into all 64 lanes of
+// a row from the scalar identity C = (G + GP + cin) ^ G ^ GP.  A wave fixes
+// its carry-in from the rows just below its segment: the highest lane there
+// that does not propagate (it generates or kills a carry) decides the row's
+// carry-out, whatever came from further below.  Up to kAddWarm rows are
+// examined, nearest first; the start of the stream has carry 0.  A segment
+// whose examined rows all propagate speculates carry-in 0 and is marked;
+// the fix-up launch walks the marked segments in order.  No atomics on a
+// shared address, no waits, no cross-wave dependency.
+__device__ __forceinline__ uint64_t sum_with_carries(uint64_t s, uint64_t G, uint64_t GP, unsigned cin, unsigned lane,
+                                                     unsigned* cout) {
+    const uint64_t t = G + GP;
+    const unsigned c1 = t < G;
+    const uint64_t u = t + (uint64_t)cin;
+    *cout = c1 | (unsigned)(u < t);
+    return s + (((u ^ G ^ GP) >> lane) & 1ull);
+}
+
 __global__ __launch_bounds__(256) void k_add(uint64_t* __restrict__ dst, const uint64_t* __restrict__ a,
-                                             const uint64_t* __restrict__ b, uint64_t n, uint64_t lw,
+                                             const uint64_t* __restrict__ b, uint64_t n_rows, uint64_t lw,
                                              uint64_t lastmask, unsigned* __restrict__ ctl,
-                                             unsigned* __restrict__ status, unsigned n_tiles) {
-    __shared__ unsigned s_tile, s_epoch, s_cin;
-    __shared__ unsigned s_wf[4];
+                                             unsigned* __restrict__ seg_carry, unsigned* __restrict__ unproven,
+                                             uint64_t n_segs) {
     const unsigned lane = lane_id();
-    const unsigned wave = threadIdx.x >> 6;
-    if (threadIdx.x == 0) {
-        s_tile = atomicAdd(&ctl[0], 1u);
-        s_epoch = __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    const unsigned tile = s_tile;
-    const unsigned tag = ((s_epoch & 0x0fffffffu) + 1u) << 4;
-    const uint64_t base = (uint64_t)tile * 1024 + (uint64_t)wave * 256;
-
-    uint64_t sum[4], G[4], GP[4];
-    unsigned f0 = 0, f1 = 1;
+    const uint64_t seg = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (seg >= n_segs) return;
+    const uint64_t r0 = seg * AddScanState::kRows;
+    const uint64_t nr = n_rows - r0 < (uint64_t)AddScanState::kRows ? n_rows - r0 : (uint64_t)AddScanState::kRows;
+    uint64_t x[AddScanState::kRows], y[AddScanState::kRows];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint64_t w = base + (uint64_t)r * 64 + lane;
-        const uint64_t x = w < n ? a[w] : 0ull;
-        const uint64_t y = w < n ? b[w] : 0ull;
-        sum[r] = x + y;
-        G[r] = ballot(sum[r] < x);
-        GP[r] = G[r] | ballot(sum[r] == ~0ull);
-        f0 = carry_out(G[r], GP[r], f0);
-        f1 = carry_out(G[r], GP[r], f1);
+    for (int r = 0; r < AddScanState::kRows; ++r) {
+        const uint64_t w = (r0 + r) * 64 + lane;
+        x[r] = (uint64_t)r < nr ? a[w] : 0ull;
+        y[r] = (uint64_t)r < nr ? b[w] : 0ull;
     }
-    if (lane == 0) s_wf[wave] = f0 | (f1 << 1);
-    __syncthreads();
-
-    if (threadIdx.x == 0) {
-        unsigned t0 = 0, t1 = 1;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const unsigned wf = s_wf[v];
-            t0 = (wf >> t0) & 1u;
-            t1 = (wf >> t1) & 1u;
-        }
-        unsigned cin = 0;
-        if (tile > 0) {
-            __hip_atomic_store(&status[tile], tag | (1u << 2) | t0 | (t1 << 1), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            unsigned h0 = 0, h1 = 1;  // carry into this tile as a function of carry into tile j+1
-            for (int j = (int)tile - 1;; --j) {
-                unsigned st;
-                for (;;) {
-                    st = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if ((st & ~15u) == tag && (st & 12u)) break;
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                if (((st >> 2) & 3u) == 2u) { cin = (st & 1u) ? h1 : h0; break; }
-                const unsigned g0 = st & 1u, g1 = (st >> 1) & 1u;
-                const unsigned n0 = g0 ? h1 : h0, n1 = g1 ? h1 : h0;
-                h0 = n0; h1 = n1;
-                if (h0 == h1) { cin = h0; break; }
+    unsigned cin = 0;
+    bool proven = true;
+    if (r0 > 0) {
+        proven = false;
+        for (int k = 1; k <= AddScanState::kWarm && !proven; ++k) {
+            if ((uint64_t)k > r0) { proven = true; break; }  // the stream starts below: carry 0
+            const uint64_t w = (r0 - k) * 64 + lane;
+            const uint64_t xa = a[w], s = xa + b[w];
+            const uint64_t G = ballot(s < xa);
+            const uint64_t NP = ballot(s != ~0ull);  // lanes that generate or kill
+            if (NP) {
+                cin = (unsigned)((G >> (63 - __builtin_clzll(NP))) & 1ull);
+                proven = true;
             }
         }
-        __hip_atomic_store(&status[tile], tag | (2u << 2) | (cin ? t1 : t0), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        s_cin = cin;
     }
-    __syncthreads();
-
-    unsigned cin = s_cin;
-    for (unsigned v = 0; v < wave; ++v) cin = (s_wf[v] >> cin) & 1u;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint64_t w = base + (uint64_t)r * 64 + lane;
-        const uint64_t t = G[r] + GP[r];
-        const unsigned c1 = t < G[r];
-        const uint64_t u = t + (uint64_t)cin;
-        const unsigned c2 = u < t;
-        const uint64_t C = u ^ G[r] ^ GP[r];       // carry into each lane
-        const uint64_t v = sum[r] + ((C >> lane) & 1ull);
-        cin = c1 | c2;
-        if (w < n) dst[w] = clamp_word(v, w, lw, lastmask);
+    for (int r = 0; r < AddScanState::kRows; ++r) {
+        if ((uint64_t)r >= nr) break;
+        const uint64_t w = (r0 + r) * 64 + lane;
+        const uint64_t s = x[r] + y[r];
+        const uint64_t G = ballot(s < x[r]);
+        const uint64_t GP = G | ballot(s == ~0ull);
+        unsigned cout;
+        const uint64_t v = sum_with_carries(s, G, GP, cin, lane, &cout);
+        cin = cout;
+        dst[w] = clamp_word(v, w, lw, lastmask);
     }
-
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        // Every block took its ticket before it counts itself done, so the
-        // last one can reset the counters; statuses are device-scope atomics
-        // and need no cache writeback.
-        if (atomicAdd(&ctl[1], 1u) == n_tiles - 1) {
-            __hip_atomic_store(&ctl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ctl[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&ctl[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0) {
+        seg_carry[seg] = cin;
+        if (!proven) {
+            __hip_atomic_fetch_or(&unproven[seg >> 5], 1u << (seg & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+}
+
+// Fix-up of k_add (one wave): ends at once when no segment was marked;
+// otherwise walks the marked segments in increasing order, gives each the
+// carry-out of the segment below (exact: proven by its own warm-up, or
+// corrected earlier in this walk), recomputes it when that carry is 1, and
+// clears the marks for the next launch.
+__global__ __launch_bounds__(64) void k_add_fix(uint64_t* __restrict__ dst, const uint64_t* __restrict__ a,
+                                                const uint64_t* __restrict__ b, uint64_t n_rows, uint64_t lw,
+                                                uint64_t lastmask, unsigned* __restrict__ ctl,
+                                                unsigned* __restrict__ seg_carry, unsigned* __restrict__ unproven,
+                                                uint64_t n_segs) {
+    const unsigned lane = lane_id();
+    if (__hip_atomic_load(&ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+    const uint64_t nbw = (n_segs + 31) / 32;
+    uint64_t last_seg = ~0ull;
+    unsigned last_carry = 0;
+    for (uint64_t wb = 0; wb < nbw; wb += 64) {
+        const uint64_t wi = wb + lane;
+        unsigned bits = wi < nbw ? unproven[wi] : 0u;
+        if (wi < nbw) unproven[wi] = 0u;
+        uint64_t hit = ballot(bits != 0u);
+        while (hit) {
+            const unsigned l = (unsigned)__builtin_ctzll(hit);
+            hit &= hit - 1;
+            unsigned word = (unsigned)__builtin_amdgcn_readlane((int)bits, (int)l);
+            while (word) {
+                const uint64_t seg = (wb + l) * 32 + (unsigned)__builtin_ctz(word);
+                word &= word - 1;
+                unsigned cin = seg == last_seg + 1 ? last_carry : seg_carry[seg - 1];
+                if (cin) {
+                    const uint64_t r0 = seg * AddScanState::kRows;
+                    for (uint64_t r = r0; r < r0 + AddScanState::kRows && r < n_rows; ++r) {
+                        const uint64_t w = r * 64 + lane;
+                        const uint64_t xa = a[w], s = xa + b[w];
+                        const uint64_t G = ballot(s < xa);
+                        const uint64_t GP = G | ballot(s == ~0ull);
+                        unsigned cout;
+                        const uint64_t v = sum_with_carries(s, G, GP, cin, lane, &cout);
+                        cin = cout;
+                        dst[w] = clamp_word(v, w, lw, lastmask);
+                    }
+                    if (lane == 0) seg_carry[seg] = cin;
+                } else {
+                    cin = seg_carry[seg];  // speculation was right: the stored carry-out stands
+                }
+                last_seg = seg;
+                last_carry = cin;
+            }
+        }
+    }
+    if (lane == 0) __hip_atomic_store(&ctl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
@@ -209,20 +235,28 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_popcount(unsigned long long* __restrict__ slot,
+// Bounded popcount: 16 bytes per lane per step, v_bcnt, a wave reduction,
+// then one atomic add per block into slot (block mod kPopSlots) of the
+// accumulator, slots 128 bytes apart, so no address takes more than a few.
+__global__ __launch_bounds__(256) void k_popcount(unsigned long long* __restrict__ slots,
                                                   const uint64_t* __restrict__ src, uint64_t lw,
                                                   uint64_t lastmask) {
     __shared__ uint64_t part[4];
     uint64_t acc = 0;
+    const uint64_t pairs = (lw + 1) / 2;   // the buffer is row-padded: word lw (if odd) exists and is 0
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < lw; w += stride)
-        acc += (uint64_t)__popcll(w + 1 == lw ? (src[w] & lastmask) : src[w]);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < pairs; i += stride) {
+        const ulonglong2 v = reinterpret_cast<const ulonglong2*>(src)[i];
+        const uint64_t w = 2 * i;
+        acc += (uint64_t)__popcll(w + 1 == lw ? (v.x & lastmask) : v.x);
+        if (w + 1 < lw) acc += (uint64_t)__popcll(w + 2 == lw ? (v.y & lastmask) : v.y);
+    }
     acc = wave_sum(acc);
     if (lane_id() == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t t = part[0] + part[1] + part[2] + part[3];
-        if (t) atomicAdd(slot, (unsigned long long)t);
+        const uint64_t t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(&slots[(blockIdx.x % kPopSlots) * kPopSlotStride], (unsigned long long)t);
     }
 }
 
@@ -309,19 +343,33 @@ void launch_shr(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t k, c
 }
 
 void launch_add(hipStream_t s, uint64_t* dst, const uint64_t* a, const uint64_t* b, const Geom& g, void* scan_state) {
-    const uint64_t tiles = AddScanState::tiles(g);
-    if (tiles > 0xffffffffull) throw HipError("stream too long for one add launch");
+    const uint64_t segs = AddScanState::segments(g);
+    const uint64_t blocks = (segs + 3) / 4;
+    if (blocks > 0x7fffffffull) throw HipError("stream too long for one add launch");
     unsigned* ctl = reinterpret_cast<unsigned*>(scan_state);
-    unsigned* status = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(scan_state) + 64);
-    k_add<<<(unsigned)tiles, 256, 0, s>>>(dst, a, b, g.alloc_words(), g.lw, g.lastmask, ctl, status, (unsigned)tiles);
+    unsigned* carry = ctl + 16;
+    unsigned* unproven = carry + segs;
+    k_add<<<(unsigned)blocks, 256, 0, s>>>(dst, a, b, g.rows, g.lw, g.lastmask, ctl, carry, unproven, segs);
+    check_launch();
+    k_add_fix<<<1, 64, 0, s>>>(dst, a, b, g.rows, g.lw, g.lastmask, ctl, carry, unproven, segs);
     check_launch();
 }
 
-void launch_popcount(hipStream_t s, unsigned long long* slot, const uint64_t* src, const Geom& g) {
+void launch_popcount(hipStream_t s, unsigned long long* slots, const uint64_t* src, const Geom& g) {
     if (g.bitlength <= 0) return;  // bsim: popcount of an unbounded case is 0 (cuda_backend.cuh:88)
-    const unsigned grid = std::min<unsigned>(2048u, blocks_for(g.lw, 256));
-    k_popcount<<<grid, 256, 0, s>>>(slot, src, g.lw, g.lastmask);
+    // about 8 words per lane: enough bytes in flight, few atomics
+    const unsigned grid = std::max(1u, std::min<unsigned>(4096u, blocks_for((g.lw + 1) / 2, 256 * 4)));
+    k_popcount<<<grid, 256, 0, s>>>(slots, src, g.lw, g.lastmask);
     check_launch();
+}
+
+uint64_t read_popcount_slots(hipStream_t s, const unsigned long long* slots) {
+    std::vector<unsigned long long> h((size_t)kPopSlots * kPopSlotStride);
+    BSZ_HIP(hipMemcpyAsync(h.data(), slots, h.size() * 8, hipMemcpyDeviceToHost, s));
+    BSZ_HIP(hipStreamSynchronize(s));
+    uint64_t t = 0;
+    for (int i = 0; i < kPopSlots; ++i) t += h[(size_t)i * kPopSlotStride];
+    return t;
 }
 
 void launch_nonzero(hipStream_t s, unsigned long long* slot, const uint64_t* src, const Geom& g) {

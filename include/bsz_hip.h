@@ -171,6 +171,9 @@ int bsz_plan_replay(bsz_plan* plan, int times);
 int bsz_plan_read_int(bsz_plan* plan, int inst, const char* name, int64_t* out);
 /* Chunks whose speculative start state was wrong in the last run, kernel k. */
 int bsz_plan_mismatches(bsz_plan* plan, int kernel, int64_t* out);
+/* Chunks of kernel k whose warm-up rows did not prove their start state
+ * exact in the last run (only these are checked by the fix-up launch). */
+int bsz_plan_unproven(bsz_plan* plan, int kernel, int64_t* out);
 /* BSZ_EXEC_FUSED (default) or BSZ_EXEC_PEROP: one kernel per DSL op over the
  * same recorded DAG and the same bound buffers (the plain comparator). */
 int bsz_plan_set_executor(bsz_plan* plan, int executor);

@@ -51,3 +51,17 @@ def assert_same(hip: dict, ora: dict, bitlength: int, int_names, label=""):
             assert h == v, f"{label}: int {k}: {h} != {v}"
         else:
             assert h & mask == v & mask, f"{label}: stream {k}"
+
+
+def plan_or_refusal(c: api.Context, program: api.Program, shape: api.Case, options: dict):
+    """A plan, or None when the library refuses the schedule because its
+    compiled kernels would use AGPRs or scratch.  Only schedules that ask
+    for occupancy 1 (the whole register file) may be refused: automatic
+    occupancy keeps every kernel within 256 VGPRs."""
+    from bitstreamzoo_amd import tuning
+    try:
+        return api.Plan(c, program, shape, options)
+    except api.BszError as e:
+        if tuning.refused(e) and options.get("occupancy") == 1:
+            return None
+        raise

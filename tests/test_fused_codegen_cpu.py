@@ -159,18 +159,40 @@ def test_string_results_come_from_one_complete_call():
     assert _fill_str(call, cap=4) == '{"a":"' + "x" * 70000 + '"}'
 
 
-def test_schedule_that_overflows_vgprs_is_refused():
+def test_overflowing_kernel_is_regenerated_smaller():
     """Two rows of the adder's 65 inputs in flight at occupancy 1 compile to a
-    kernel that uses AGPRs; such kernels gave wrong high halves of 64-bit
-    values on MI355X, so the library refuses the schedule (RUNTIME error)
-    instead of running it.  The same adder at occupancy 2 is accepted."""
+    kernel that uses AGPRs (kernels like it gave wrong high halves of 64-bit
+    values on MI355X).  The library never runs such a kernel: it regenerates
+    it with fewer rows per step, here one, which fits the 256 VGPRs."""
     w = workloads.adder32(1 << 16, seed=11)
     prog, case = api.Program(w.source), api.Case.from_bscase(w.cases[0])
+    info = api.compile_plan_only(prog, case, {"chunk_rows": 8, "rows_per_step": 2, "occupancy": 1})
+    (k,) = info["kernels"]
+    assert k["rows_per_step"] == 1 and k["agprs"] == 0 and k["scratch_bytes"] == 0 and 0 < k["vgprs"] <= 256, k
+
+
+def test_default_schedule_never_spills():
+    """The batched Shift-And at the default schedule (8 rows per step at the
+    estimated occupancy) compiles to a kernel that spills to scratch; the
+    regenerated kernel issues fewer rows per step and has no scratch."""
+    w = workloads.shift_and_batch(n_seqs=4, seed=5, min_log2=8, max_log2=10)
+    from bitstreamzoo_amd import tuning
+    info = api.compile_plan_only(api.Program(w.source), tuning.shape_case(w.cases), {})
+    assert all(k["agprs"] == 0 and k["scratch_bytes"] == 0 and k["vgprs"] > 0 for k in info["kernels"]), info
+    assert info["kernels"][0]["rows_per_step"] < 8
+
+
+def test_kernel_that_cannot_shrink_is_refused():
+    """A wide random-DAG partition (max_live 160) at an explicit occupancy 1
+    and one row per step still needs AGPRs: nothing is left to shrink, so
+    the schedule is refused with a RUNTIME error instead of being run."""
+    from bitstreamzoo_amd import dag_gen, tuning
+    p = dag_gen.generate(1200, seed=2)
+    case = dag_gen.text_case(5000, seed=2)
     with pytest.raises(BszError) as e:
-        api.compile_plan_only(prog, case, {"chunk_rows": 8, "rows_per_step": 2, "occupancy": 1})
+        api.compile_plan_only(api.Program(p.source), tuning.shape_case([case]),
+                              {"chunk_rows": 8, "rows_per_step": 1, "max_live": 160, "occupancy": 1})
     assert e.value.code == -3 and "schedule refused" in str(e.value) and "AGPRs" in str(e.value)
-    info = api.compile_plan_only(prog, case, {"chunk_rows": 4, "rows_per_step": 1, "occupancy": 2})
-    assert all(k["agprs"] == 0 and k["scratch_bytes"] == 0 and k["vgprs"] > 0 for k in info["kernels"])
 
 
 def test_register_check_fails_closed():

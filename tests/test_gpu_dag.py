@@ -7,7 +7,7 @@ import pytest
 from bitstreamzoo_amd import api, dag_gen, tuning
 from bitstreamzoo_amd.bsdata import words_to_int
 from cpu_models import Oracle
-from hip_models import ctx
+from hip_models import ctx, plan_or_refusal
 from runner import run_model
 
 pytestmark = pytest.mark.gpu
@@ -29,8 +29,10 @@ def test_dag_plan_matches_oracle(seed, n_bytes, sched):
     case = dag_gen.text_case(n_bytes, seed=seed)
     exp = oracle_outputs(p, case)
     c = ctx()
+    plan = plan_or_refusal(c, api.Program(p.source), tuning.shape_case([case]), SCHEDULES[sched])
+    if plan is None:
+        return  # refused: its kernels would overflow the VGPRs (never run)
     res = tuning.Resident(c, [case])
-    plan = api.Plan(c, api.Program(p.source), tuning.shape_case([case]), SCHEDULES[sched])
     res.bind(plan)
     plan.run()
     nw = (n_bytes + 63) // 64
@@ -73,10 +75,12 @@ def test_rows_per_step_sweep_wide_kernel(chunk_rows, rows_per_step, occ):
     case = dag_gen.text_case(64 * 64 * 40 + 123, seed=spec.seed + 7)
     exp = oracle_outputs(p, case)
     c = ctx()
-    res = tuning.Resident(c, [case])
     opts = {"chunk_rows": chunk_rows, "rows_per_step": rows_per_step, "occupancy": occ, "max_live": 160,
             "max_ops": 4000}
-    plan = api.Plan(c, api.Program(p.source), tuning.shape_case([case]), opts)
+    plan = plan_or_refusal(c, api.Program(p.source), tuning.shape_case([case]), opts)
+    if plan is None:
+        return  # refused: its kernels would overflow the VGPRs (never run)
+    res = tuning.Resident(c, [case])
     res.bind(plan)
     plan.run()
     nw = (case.bitlength + 63) // 64
@@ -105,8 +109,10 @@ def test_wide_schedules_match_oracle_without_spills(args, idx, sched):
     case = dag_gen.text_case(min(sp.n_bytes, 400_000), seed=sp.seed + 7)
     exp = oracle_outputs(p, case)
     c = ctx()
+    plan = plan_or_refusal(c, api.Program(p.source), tuning.shape_case([case]), sched)
+    if plan is None:
+        return  # refused: its kernels would overflow the VGPRs (never run)
     res = tuning.Resident(c, [case])
-    plan = api.Plan(c, api.Program(p.source), tuning.shape_case([case]), sched)
     res.bind(plan)
     plan.run()
     nw = (case.bitlength + 63) // 64

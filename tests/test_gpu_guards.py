@@ -16,27 +16,34 @@ from hip_models import ctx as _ctx
 
 pytestmark = pytest.mark.gpu
 
-UNCLEAN = {"chunk_rows": 8, "rows_per_step": 2, "occupancy": 1}
+# an explicit occupancy 1 and one row per step: a wide partition of this
+# schedule still needs AGPRs, so nothing is left to shrink
+UNCLEAN = {"chunk_rows": 8, "rows_per_step": 1, "max_live": 160, "occupancy": 1}
+
+
+def _dag():
+    from bitstreamzoo_amd import dag_gen
+    return dag_gen.generate(1200, seed=2), dag_gen.text_case(5000, seed=2)
 
 
 def test_plan_create_refuses_overflowing_schedule():
     ctx = _ctx()
-    w = workloads.adder32(1 << 16, seed=11)
+    p, case = _dag()
     with pytest.raises(api.BszError) as e:
-        api.Plan(ctx, api.Program(w.source), tuning.shape_case(w.cases), UNCLEAN)
+        api.Plan(ctx, api.Program(p.source), tuning.shape_case([case]), UNCLEAN)
     assert e.value.code == -3 and "schedule refused" in str(e.value)
 
 
 def test_cli_refuses_overflowing_schedule(tmp_path):
-    w = workloads.adder32(4096, seed=3)
-    bs = tmp_path / "adder.bs"
-    bs.write_text(w.source)
-    data = tmp_path / "adder.bsdata"
-    c = w.cases[0]
-    write_bsdata(str(data), [{"bitlength": c.bitlength, "inputs": dict(c.inputs)}])
-    env = dict(os.environ, BSZ_OCCUPANCY="1", BSZ_ROWS_PER_STEP="2", BSZ_CHUNK_ROWS="8")
+    p, c = _dag()
+    bs = tmp_path / "dag.bs"
+    bs.write_text(p.source)
+    data = tmp_path / "dag.bsdata"
+    write_bsdata(str(data), [{"bitlength": c.bitlength, "inputs": dict(c.inputs),
+                              "input_arrays": {k: dict(v) for k, v in c.input_arrays.items()}}])
+    env = dict(os.environ, BSZ_OCCUPANCY="1", BSZ_ROWS_PER_STEP="1", BSZ_CHUNK_ROWS="8", BSZ_MAX_LIVE="160")
     r = subprocess.run([_lib.CLI_PATH, str(bs), "--input", str(data)], capture_output=True, text=True, env=env,
-                       timeout=120)
+                       timeout=300)
     assert r.returncode == 1 and "schedule refused" in r.stderr, (r.returncode, r.stderr[-500:])
 
 
