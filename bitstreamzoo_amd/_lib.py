@@ -40,6 +40,7 @@ _SIGS = {
     "bsz_op_or": (C.c_int, [vp, u64p, u64p, u64p, C.c_int64]),
     "bsz_op_xor": (C.c_int, [vp, u64p, u64p, u64p, C.c_int64]),
     "bsz_op_add": (C.c_int, [vp, u64p, u64p, u64p, C.c_int64]),
+    "bsz_op_add_noalloc": (C.c_int, [vp, u64p, u64p, u64p, C.c_int64]),
     "bsz_op_shl": (C.c_int, [vp, u64p, u64p, C.c_uint64, C.c_int64]),
     "bsz_op_shr": (C.c_int, [vp, u64p, u64p, C.c_uint64, C.c_int64]),
     "bsz_op_popcount": (C.c_int, [vp, u64p, C.c_int64, i64p]),

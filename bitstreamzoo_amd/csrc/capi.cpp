@@ -73,6 +73,8 @@ int guard(bsz_ctx* ctx, F&& f) {
         return fail(ctx, m.find("out of memory") != std::string::npos ? BSZ_E_OOM : BSZ_E_HIP, m);
     } catch (const std::bad_alloc&) {
         return fail(ctx, BSZ_E_OOM, "host out of memory");
+    } catch (const std::invalid_argument& e) {
+        return fail(ctx, BSZ_E_INVALID, e.what());
     } catch (const std::exception& e) {
         return fail(ctx, BSZ_E_RUNTIME, e.what());
     }
@@ -108,7 +110,10 @@ int bsz_ctx_create(bsz_ctx** out, int device) {
     auto* c = new (std::nothrow) bsz_ctx;
     if (!c) return fail(nullptr, BSZ_E_OOM, "host out of memory");
     c->device = device;
-    int rc = guard(c, [&] { BSZ_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking)); });
+    int rc = guard(c, [&] {
+        BSZ_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+        setup_noalloc_scratch(device);  // once per device, with the device current
+    });
     if (rc != BSZ_OK) { t_err = c->err; delete c; return rc; }
     *out = c;
     return BSZ_OK;
@@ -199,6 +204,13 @@ int bsz_op_add(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b
         Geom g = geom_of(bl);
         ensure_scalar_state(ctx, g);
         launch_add(ctx->s, dst, a, b, g, ctx->add_state);
+    });
+}
+int bsz_op_add_noalloc(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bl) {
+    return guard(ctx, [&] {
+        if (dst == a || dst == b) throw std::invalid_argument("bsz_op_add_noalloc: dst aliases an operand");
+        capture_gate(ctx);
+        launch_add_noalloc(ctx->s, ctx->device, dst, a, b, geom_of(bl));
     });
 }
 int bsz_op_shl(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, uint64_t k, int64_t bl) {

@@ -100,6 +100,14 @@ struct AddScanState {
     static size_t bytes(const Geom& g) { return 64 + segments(g) * 4 + (segments(g) + 31) / 32 * 4; }
 };
 void launch_add(hipStream_t s, uint64_t* dst, const uint64_t* a, const uint64_t* b, const Geom& g, void* scan_state);
+// The same addition with its scratch in a fixed device array of the library
+// (no allocation): streams of at most kNoAllocMaxBits bits.  The array is
+// shared by every call on a device, so such calls must not overlap there.
+// setup_noalloc_scratch(device) must run once with `device` current.
+constexpr uint64_t kNoAllocMaxBits = 1ull << 34;
+void setup_noalloc_scratch(int device);
+void launch_add_noalloc(hipStream_t s, int device, uint64_t* dst, const uint64_t* a, const uint64_t* b,
+                        const Geom& g);
 // A popcount accumulator: kPopSlots u64 slots kPopSlotStride words apart
 // (kPopSlotWords words in all, zeroed by the caller before the launch).
 constexpr int kPopSlots = 64;

@@ -1,7 +1,9 @@
 // executor.cpp — program walk (control plane) and the one-kernel-per-op engine.
 #include "executor.h"
 
+#include <algorithm>
 #include <chrono>
+#include <set>
 
 namespace bsz {
 
@@ -27,6 +29,7 @@ public:
     RunResult run() {
         walk();
         E.flush();
+        settle_predicated();
         return collect();
     }
 
@@ -74,7 +77,7 @@ private:
         RunResult r;
         for (Sym o : P.outputs) {
             const std::string& nm = P.syms.names[o];
-            if (vars[o]) {
+            if (vars[o] && !absent_vars.count(o)) {
                 std::vector<uint64_t> w(g.lw);
                 E.download(vars[o], w.data());
                 r.streams[nm] = std::move(w);
@@ -83,6 +86,7 @@ private:
             if (ai != arrays.end()) {
                 auto& dst = r.arrays[nm];
                 for (auto& e : ai->second) {
+                    if (absent_elems.count({o, e.first})) continue;
                     std::vector<uint64_t> w(g.lw);
                     E.download(e.second, w.data());
                     dst[e.first] = std::move(w);
@@ -100,7 +104,7 @@ private:
         int64_t nv = 0;
         for (auto& v : vars) nv += v ? 1 : 0;
         for (auto& a : arrays) nv += (int64_t)a.second.size();
-        r.n_variables = nv;
+        r.n_variables = nv - (int64_t)absent_vars.size() - (int64_t)absent_elems.size();
         return r;
     }
 
@@ -125,6 +129,181 @@ private:
 
     void count(OpIdx i) { ++ops; ++mix[i]; }
 
+    // ── device-side `if` (if-conversion) ──
+    // An `if` whose body only assigns streams (no int assignments, loops or
+    // declarations of arrays) runs on the device when the engine supports
+    // it: the body is walked unconditionally, every stream it assigns
+    // becomes select(p, new, old), and the ops it counts, the variables and
+    // elements it creates, count only where every enclosing predicate holds,
+    // which the host learns once, after the final flush.  A body that raises
+    // a run-time error is undone and the `if` is decided on the host instead
+    // (the reference raises only where the body runs).
+    struct Frame {
+        PopRef pred;
+        std::vector<std::pair<Sym, SV>> old_vars;                      // first old value (null: did not exist)
+        std::vector<std::tuple<Sym, int64_t, SV, bool>> old_elems;     // (array, idx, old, existed)
+        std::unordered_map<Sym, char> seen_vars;
+        std::set<std::pair<Sym, int64_t>> seen_elems;
+        size_t cond_created0 = 0, cond_ops0 = 0;
+        int64_t ops0 = 0;
+        int64_t mix0[OP_N] = {};
+    };
+    struct CondOps {
+        std::vector<PopRef> preds;
+        int64_t ops = 0;
+        int64_t mix[OP_N] = {};
+    };
+    struct Created {   // a variable or element first created inside predicated frames
+        Sym sym;
+        int64_t idx;   // -1: a scalar variable
+        std::vector<PopRef> preds;
+    };
+    std::vector<Frame> frames;
+    std::vector<CondOps> cond_ops;
+    std::vector<Created> cond_created;
+    std::set<Sym> absent_vars;
+    std::set<std::pair<Sym, int64_t>> absent_elems;
+
+    std::vector<PopRef> pred_stack() const {
+        std::vector<PopRef> v;
+        for (const Frame& f : frames) v.push_back(f.pred);
+        return v;
+    }
+
+    // Before vars[s] changes (or is created) inside predicated frames.
+    void note_var(Sym s) {
+        for (Frame& f : frames) {
+            if (f.seen_vars.emplace(s, 1).second) f.old_vars.emplace_back(s, vars[s]);
+        }
+        if (!frames.empty() && !vars[s]) {
+            cond_created.push_back({s, -1, pred_stack()});
+            if ((size_t)s >= cond_only.size()) cond_only.resize(vars.size(), 0);
+            cond_only[s] = 1;
+        }
+    }
+    void note_elem(Sym a, int64_t idx, bool exists, const SV& cur) {
+        for (Frame& f : frames) {
+            if (f.seen_elems.insert({a, idx}).second) f.old_elems.emplace_back(a, idx, cur, exists);
+        }
+        if (!frames.empty() && !exists) cond_created.push_back({a, idx, pred_stack()});
+    }
+
+    static bool convertible(const std::vector<Stmt>& body) {
+        for (const Stmt& b : body) {
+            switch (b.k) {
+                case Stmt::Assign: case Stmt::ArrAssign: case Stmt::Local: break;
+                case Stmt::If: if (!convertible(b.body)) return false; break;
+                default: return false;
+            }
+        }
+        return true;
+    }
+
+    void exec_if(const Stmt& s) {
+        SV c = var(P.sex[s.e].name);
+        if (c == zero_c || g.bitlength <= 0) {  // never true on the device-side path's terms
+            if (E.is_nonzero(c)) for (const Stmt& b : s.body) exec(b);
+            return;
+        }
+        if (!E.supports_predicates() || !convertible(s.body) || c == ones_c) {
+            if (E.is_nonzero(c)) for (const Stmt& b : s.body) exec(b);
+            return;
+        }
+        Frame f;
+        f.pred = E.predicate(c);
+        f.ops0 = ops;
+        std::copy(mix, mix + OP_N, f.mix0);
+        f.cond_created0 = cond_created.size();
+        f.cond_ops0 = cond_ops.size();
+        frames.push_back(std::move(f));
+        try {
+            for (const Stmt& b : s.body) exec(b);
+        } catch (const RunError&) {
+            undo_frame();
+            if (E.is_nonzero(c)) for (const Stmt& b : s.body) exec(b);
+            return;
+        }
+        Frame fr = std::move(frames.back());
+        // ops counted directly in this body (nested bodies already moved theirs out)
+        CondOps co;
+        co.preds = pred_stack();
+        co.ops = ops - fr.ops0;
+        for (int i = 0; i < OP_N; ++i) co.mix[i] = mix[i] - fr.mix0[i];
+        cond_ops.push_back(std::move(co));
+        ops = fr.ops0;
+        std::copy(fr.mix0, fr.mix0 + OP_N, mix);
+        frames.pop_back();
+        for (auto& ov : fr.old_vars) {
+            Sym sv = ov.first;
+            SV now = vars[sv];
+            SV merged = E.select(fr.pred, now ? now : zero_c, ov.second ? ov.second : zero_c);
+            for (Frame& outer : frames)  // the merge is an assignment of the enclosing bodies
+                if (outer.seen_vars.emplace(sv, 1).second) outer.old_vars.emplace_back(sv, ov.second);
+            vars[sv] = merged;
+        }
+        for (auto& oe : fr.old_elems) {
+            Sym a = std::get<0>(oe);
+            int64_t idx = std::get<1>(oe);
+            const SV& old = std::get<2>(oe);
+            SV& slot = arrays[a][idx];
+            SV merged = E.select(fr.pred, slot ? slot : zero_c, old ? old : zero_c);
+            for (Frame& outer : frames)
+                if (outer.seen_elems.insert({a, idx}).second) outer.old_elems.emplace_back(a, idx, old, std::get<3>(oe));
+            slot = merged;
+        }
+    }
+
+    // Restores everything the innermost frame changed and drops it.
+    void undo_frame() {
+        Frame fr = std::move(frames.back());
+        frames.pop_back();
+        for (auto it = fr.old_vars.rbegin(); it != fr.old_vars.rend(); ++it) vars[it->first] = it->second;
+        for (auto it = fr.old_elems.rbegin(); it != fr.old_elems.rend(); ++it) {
+            Sym a = std::get<0>(*it);
+            int64_t idx = std::get<1>(*it);
+            if (std::get<3>(*it)) arrays[a][idx] = std::get<2>(*it);
+            else {
+                auto ai = arrays.find(a);
+                if (ai != arrays.end()) ai->second.erase(idx);
+            }
+        }
+        cond_created.resize(fr.cond_created0);
+        cond_ops.resize(fr.cond_ops0);  // records of nested bodies that completed inside this one
+        ops = fr.ops0;
+        std::copy(fr.mix0, fr.mix0 + OP_N, mix);
+    }
+
+    bool all_true(const std::vector<PopRef>& ps) {
+        for (const PopRef& p : ps)
+            if (E.resolve(p) == 0) return false;
+        return true;
+    }
+
+    // After the final flush: conditional op counts, and which conditionally
+    // created variables and elements exist.
+    void settle_predicated() {
+        for (const CondOps& co : cond_ops) {
+            if (!all_true(co.preds)) continue;
+            ops += co.ops;
+            for (int i = 0; i < OP_N; ++i) mix[i] += co.mix[i];
+        }
+        std::map<std::pair<Sym, int64_t>, bool> exists;
+        for (const Created& c : cond_created) {
+            bool& e = exists[{c.sym, c.idx}];
+            e = e || all_true(c.preds);
+        }
+        for (auto& kv : exists) {
+            if (kv.second || unconditional.count(kv.first)) continue;
+            if (kv.first.second < 0) absent_vars.insert(kv.first.first);
+            else absent_elems.insert(kv.first);
+        }
+    }
+    // variables / elements written or created outside any predicated body
+    // (they exist for certain), and the variables created only inside one so
+    // far (a later unconditional read makes them exist too)
+    std::set<std::pair<Sym, int64_t>> unconditional;
+    std::vector<char> cond_only;
+
     int64_t value_of(IntSlot& is) {
         if (is.pop) {
             is.v = E.resolve(is.pop);
@@ -136,9 +315,21 @@ private:
     const std::string& nm(Sym s) const { return P.syms.names[s]; }
 
     SV& var(Sym s) {
-        SV& v = vars[s];
-        if (!v) v = zero_c;
-        return v;
+        if (!vars[s]) {
+            if (!frames.empty()) note_var(s);
+            else unconditional.insert({s, -1});
+            vars[s] = zero_c;
+        } else if (frames.empty() && (size_t)s < cond_only.size() && cond_only[s]) {
+            unconditional.insert({s, -1});
+            cond_only[s] = 0;
+        }
+        return vars[s];
+    }
+
+    void set_var(Sym s, SV v) {
+        if (!frames.empty()) note_var(s);
+        else unconditional.insert({s, -1});
+        vars[s] = std::move(v);
     }
 
     void load_inputs() {
@@ -266,7 +457,7 @@ private:
     void exec(const Stmt& s) {
         switch (s.k) {
             case Stmt::Local:
-                vars[s.name] = s.e >= 0 ? rhs(s.e) : zero_c;
+                set_var(s.name, s.e >= 0 ? rhs(s.e) : zero_c);
                 return;
             case Stmt::ArrDecl: {
                 int64_t n = (int32_t)eval_int(s.i1);
@@ -276,24 +467,26 @@ private:
             }
             case Stmt::Assign: {
                 var(s.name);
-                vars[s.name] = rhs(s.e);
+                SV v = rhs(s.e);
+                set_var(s.name, std::move(v));
                 return;
             }
             case Stmt::ArrAssign: {
                 int64_t idx = (int32_t)eval_int(s.i1);
                 auto& arr = arrays[s.name];
+                auto it = arr.find(idx);
+                const bool existed = it != arr.end() && it->second;
+                if (!frames.empty()) note_elem(s.name, idx, existed, existed ? it->second : SV());
+                else unconditional.insert({s.name, idx});
                 SV& slot = arr[idx];
                 if (!slot) slot = zero_c;
                 SV v = rhs(s.e);
                 arrays[s.name][idx] = v;
                 return;
             }
-            case Stmt::If: {
-                SV c = var(P.sex[s.e].name);
-                if (E.is_nonzero(c))
-                    for (const Stmt& b : s.body) exec(b);
+            case Stmt::If:
+                exec_if(s);
                 return;
-            }
             case Stmt::While: {
                 Sym cs = P.sex[s.e].name;
                 for (int it = 0; it < kMaxWhileIters; ++it) {

@@ -76,6 +76,16 @@ public:
     virtual size_t peak_bytes() const = 0;
     virtual hipStream_t stream() const = 0;
     virtual std::string stats_json() const { return "{}"; }
+    // Device-side `if` (if-conversion).  An engine that supports it keeps an
+    // `if` body on the device: predicate(c) is "c has a bit set within the
+    // bitlength", left on the device as the popcount of c (true when it is
+    // not zero), and select(p, t, f) takes t where p holds, else f, word by
+    // word.  The host reads predicates only when it needs them.
+    virtual bool supports_predicates() const { return false; }
+    virtual PopRef predicate(const SV& /*cond*/) { throw RunError("internal: this engine has no device predicates"); }
+    virtual SV select(const PopRef& /*pred*/, const SV& /*t*/, const SV& /*f*/) {
+        throw RunError("internal: this engine has no device predicates");
+    }
     // Called before input() with the DSL name (and array index, or -1).
     virtual void note_input(const std::string& /*name*/, int64_t /*idx*/) {}
 };

@@ -341,7 +341,7 @@ public:
                 hipModule_t mod;
                 BSZ_HIP(hipModuleLoadData(&mod, code.data()));
                 hipFunction_t f;
-                BSZ_HIP(hipModuleGetFunction(&f, mod, "bsz_kernel"));
+                BSZ_HIP(hipModuleGetFunction(&f, mod, k.entry.c_str()));
                 it = loaded_.emplace(h, std::make_pair(mod, f)).first;
                 regs_[h] = kernel_regs(code);
             }
@@ -530,7 +530,7 @@ public:
         for (size_t k = 0; k < K; ++k) {
             const KernelSpec& ks = P.kernels[k];
             KRes& r = kres_[k];
-            r.nin = ks.inputs.size();
+            r.nin = ks.inputs.size() + ks.preds.size();
             r.nout = ks.outputs.size();
             r.npop = ks.pops.size();
             r.nst = ks.n_state;
@@ -572,6 +572,13 @@ public:
 
     void set_in(size_t k, uint32_t inst, size_t j, const uint64_t* p) {
         in_tab_h_[k][(size_t)inst * kres_[k].nin + j] = p;
+        tables_dirty_ = true;
+    }
+    // entry j of kernel k's input table: bank 0 of the slots of popcount
+    // `pop` of instance `inst`, read as a predicate
+    void set_pred(size_t k, uint32_t inst, size_t j, size_t pop) {
+        const size_t idx = pop_index(pop, inst);
+        in_tab_h_[k][(size_t)inst * kres_[k].nin + j] = (const uint64_t*)ptr(off_pop_ + 8ull * (idx << spread_log_));
         tables_dirty_ = true;
     }
     void set_out(size_t k, uint32_t inst, size_t j, uint64_t* p) {
@@ -735,6 +742,7 @@ void bind_instance(PlanRT& rt, uint32_t inst, const std::vector<const uint64_t*>
                                                         : seg_out_by_node.at(in.id);
             rt.set_in(k, inst, j, p);
         }
+        for (size_t q = 0; q < ks.preds.size(); ++q) rt.set_pred(k, inst, ks.inputs.size() + q, (size_t)ks.preds[q]);
         for (size_t j = 0; j < ks.outputs.size(); ++j) {
             uint64_t* p = ks.out_slot[j] >= 0 ? slots.at(ks.out_slot[j]) : seg_out_by_node.at(ks.outputs[j]);
             rt.set_out(k, inst, j, p);
@@ -749,6 +757,7 @@ void bind_instance(PlanRT& rt, uint32_t inst, const std::vector<const uint64_t*>
 struct FusedPop : PopCell {
     int32_t node = -1;
     uint64_t segment = 0;
+    int32_t index = -1;   // position among the open segment's popcounts
 };
 
 struct CachedPlan {
@@ -813,8 +822,36 @@ public:
         auto p = std::make_shared<FusedPop>();
         p->node = node_of(a);
         p->segment = seg_;
+        p->index = (int32_t)pops_.size();
         pops_.push_back(p);
         return p;
+    }
+
+    bool supports_predicates() const override { return !recording_; }
+
+    PopRef predicate(const SV& c) override {
+        int32_t n = node_of(c);
+        // a predicate is accumulated by the kernel that computes its stream;
+        // an input or constant gets a computed copy so that kernel exists
+        if (nodes_[n].op == NOp::Input || nodes_[n].op == NOp::Zero || nodes_[n].op == NOp::Ones)
+            n = make(NOp::Or, n, n, 0)->node;
+        auto p = std::make_shared<FusedPop>();
+        p->node = n;
+        p->segment = seg_;
+        p->index = (int32_t)pops_.size();
+        pops_.push_back(p);
+        return p;
+    }
+
+    SV select(const PopRef& pr, const SV& t, const SV& f) override {
+        if (t == f) return t;
+        auto* p = static_cast<FusedPop*>(pr.get());
+        if (p->ready) return p->value != 0 ? t : f;  // resolved by an earlier flush
+        if (p->segment != seg_) throw RunError("internal: predicate of a closed segment is not resolved");
+        const int32_t a = node_of(t), b = node_of(f);
+        SV v = make(NOp::Sel, a, b, 0);
+        nodes_[v->node].pred = p->index;
+        return v;
     }
 
     int64_t resolve(const PopRef& p) override {
@@ -1460,7 +1497,8 @@ std::string FusedPlan::info_json() const {
     o << "{\"kernels\":[";
     for (size_t k = 0; k < I.spec->kernels.size(); ++k) {
         const KernelSpec& ks = I.spec->kernels[k];
-        o << (k ? "," : "") << "{\"name\":\"" << ks.name << "\",\"nodes\":" << ks.nodes.size()
+        o << (k ? "," : "") << "{\"name\":\"" << ks.name << "\",\"entry\":\"" << ks.entry
+          << "\",\"nodes\":" << ks.nodes.size()
           << ",\"inputs\":" << ks.inputs.size() << ",\"outputs\":" << ks.outputs.size() << ",\"pops\":" << ks.pops.size()
           << ",\"state_words\":" << ks.n_state << ",\"rows_per_step\":" << ks.rows_per_step
           << ",\"occupancy\":" << ks.occupancy << ",\"words_per_lane\":" << ks.words_per_lane

@@ -18,7 +18,7 @@ bool is_free(NOp op) { return op == NOp::Input || op == NOp::Zero || op == NOp::
 int arity(NOp op) {
     switch (op) {
         case NOp::Not: case NOp::Shl: case NOp::Shr: return 1;
-        case NOp::And: case NOp::Or: case NOp::Xor: case NOp::Add: return 2;
+        case NOp::And: case NOp::Or: case NOp::Xor: case NOp::Add: case NOp::Sel: return 2;
         default: return 0;
     }
 }
@@ -143,6 +143,13 @@ __device__ __forceinline__ u32 wave_sum(u32 v) {
   for (u32 m = 32; m > 0; m >>= 1) v += (u32)__builtin_amdgcn_ds_bpermute((int)(((l ^ m) & 63u) << 2), (int)v);
   return v;
 }
+// A predicate: the popcount slots of one (instance, popcount) in bank 0 (bank
+// 1 lies `stride` words on); lane i reads slot i of this run's bank.
+__device__ __forceinline__ bool pred_flag(const u64* slots, u64 stride, u32 ep, u32 spread_log) {
+  const u32 l = lane_id();
+  const u64 v = l < (1u << spread_log) ? ((const volatile u64*)slots)[(ep ? stride : 0ull) + l] : 0ull;
+  return ballot(v != 0ull) != 0ull;
+}
 __device__ __forceinline__ u32 aload32(u32* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT); }
 __device__ __forceinline__ u64 aload64(u64* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT); }
 __device__ __forceinline__ void astore32(u32* p, u32 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT); }
@@ -206,6 +213,9 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     for (int32_t p : ks.pops) note_operand(g.pops[p]);
     for (int32_t n : ks.outputs)
         if (!in_kernel.count(n)) note_operand(n);
+    // predicates read by this kernel's selections, each once per chunk
+    std::map<int32_t, int> pred_index;
+    for (size_t j = 0; j < ks.preds.size(); ++j) pred_index[ks.preds[j]] = (int)j;
 
     // Rows whose loads are issued together: as many as the option allows while
     // the prefetched words stay within ~96 VGPRs (2 per 64-bit word), so a
@@ -267,6 +277,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
 
     std::ostringstream s;
     const int NIN = (int)ks.inputs.size(), NOUT = (int)ks.outputs.size(), NPOP = (int)ks.pops.size();
+    const int NPRED = (int)ks.preds.size(), NINT = NIN + NPRED;  // input table: streams, then predicates
     // Register budget: at least `occupancy` waves per SIMD (512/occupancy VGPRs),
     // which also bounds how many row loads the scheduler hoists.
     // estimated VGPRs: prefetched words, popcount accumulators, live values
@@ -290,10 +301,8 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "  const u32 lane = lane_id();\n";
         s << "  const u32 off16 = lane * 16u;\n";
         s << "  const u64x2 zero2 = {0ull, 0ull};\n";
-        if (NPOP) {
-            s << "  const u32 ep = u32_at(A.ctl, 0u) & 1u;\n";
-            s << "  u64* const bank = A.pop_acc + (ep ? A.pop_stride : 0ull);\n";
-        }
+        if (NPOP || NPRED) s << "  const u32 ep = u32_at(A.ctl, 0u) & 1u;\n";
+        if (NPOP) s << "  u64* const bank = A.pop_acc + (ep ? A.pop_stride : 0ull);\n";
         s << "  const u32 c = blockIdx.x * " << WPB << "u + uni(threadIdx.x >> 6);\n";
         s << "  if (c >= A.n_chunks) return;\n";
         if (NPOP) {
@@ -305,8 +314,11 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "  const InstDesc D = inst_at(A.inst, ii);\n";
         s << "  const u32 cc = c - D.chunk_begin;\n";
         s << "  const u64 NW = D.rows * 64ull;\n";
-        for (int j = 0; j < NIN; ++j) s << "  const u64* __restrict__ in" << j << " = tab_in(A.in, (u64)ii * " << NIN << "u + " << j << "u);\n";
+        for (int j = 0; j < NIN; ++j) s << "  const u64* __restrict__ in" << j << " = tab_in(A.in, (u64)ii * " << NINT << "u + " << j << "u);\n";
         for (int j = 0; j < NOUT; ++j) s << "  u64* __restrict__ out" << j << " = tab_out(A.out, (u64)ii * " << NOUT << "u + " << j << "u);\n";
+        for (int j = 0; j < NPRED; ++j)
+            s << "  const bool pf" << j << " = pred_flag(tab_in(A.in, (u64)ii * " << NINT << "u + " << (NIN + j)
+              << "u), A.pop_stride, ep, A.spread_log);\n";
         s << "  const u64 r_begin = (u64)cc * " << CR << "ull;\n";
         s << "  const u64 r_end = (r_begin + " << CR << "ull < D.rows) ? r_begin + " << CR << "ull : D.rows;\n";
         for (int p = 0; p < NPOP; ++p) s << "  u32 pc" << p << " = 0u;\n";
@@ -361,6 +373,10 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                     case NOp::Shl: case NOp::Shr:  // only shifts by 0 reach a wide kernel
                         s << "      const u64x2 " << v << " = " << ref(x.a) << ";\n";
                         break;
+                    case NOp::Sel:
+                        s << "      const u64x2 " << v << " = pf" << pred_index.at(x.pred) << " ? " << ref(x.a) << " : "
+                          << ref(x.b) << ";\n";
+                        break;
                     default: throw std::logic_error("wide kernel with a cross-lane op");
                 }
                 sink(n);
@@ -387,11 +403,9 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     s << "  const u32 lane = lane_id();\n";
     s << "  const u32 off8 = lane * 8u;\n";
     s << "  const bool fix = A.mode != 0u;\n";
-    if (NPOP) {
-        // popcount slots of this run, and of the next one (cleared here)
-        s << "  const u32 ep = u32_at(A.ctl, 0u) & 1u;\n";
-        s << "  u64* const bank = A.pop_acc + (ep ? A.pop_stride : 0ull);\n";
-    }
+    // popcount slots of this run, and of the next one (cleared here)
+    if (NPOP || NPRED) s << "  const u32 ep = u32_at(A.ctl, 0u) & 1u;\n";
+    if (NPOP) s << "  u64* const bank = A.pop_acc + (ep ? A.pop_stride : 0ull);\n";
     s << "  u32 c = 0;\n";
     if (has_state) {
         // mode 1 (one workgroup).  Chunks whose warm-up proved their start
@@ -496,8 +510,11 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     s << "    const InstDesc D = inst_at(A.inst, ii);\n";
     s << "    const u32 cc = c - D.chunk_begin;\n";
     s << "    const u64 NW = D.rows * 64ull;\n";
-    for (int j = 0; j < NIN; ++j) s << "    const u64* __restrict__ in" << j << " = tab_in(A.in, (u64)ii * " << NIN << "u + " << j << "u);\n";
+    for (int j = 0; j < NIN; ++j) s << "    const u64* __restrict__ in" << j << " = tab_in(A.in, (u64)ii * " << NINT << "u + " << j << "u);\n";
     for (int j = 0; j < NOUT; ++j) s << "    u64* __restrict__ out" << j << " = tab_out(A.out, (u64)ii * " << NOUT << "u + " << j << "u);\n";
+    for (int j = 0; j < NPRED; ++j)
+        s << "    const bool pf" << j << " = pred_flag(tab_in(A.in, (u64)ii * " << NINT << "u + " << (NIN + j)
+          << "u), A.pop_stride, ep, A.spread_log);\n";
     s << "    const u64 r_begin = (u64)cc * " << CR << "ull;\n";
     s << "    const u64 r_end = (r_begin + " << CR << "ull < D.rows) ? r_begin + " << CR << "ull : D.rows;\n";
 
@@ -648,13 +665,17 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                     }
                     break;
                 }
+                case NOp::Sel:
+                    s << "        const u64 " << v << " = pf" << pred_index.at(x.pred) << " ? " << ref(x.a) << " : "
+                      << ref(x.b) << ";\n";
+                    break;
                 case NOp::Input: break;
             }
             if (warm) {
                 switch (x.op) {
                     case NOp::Zero: case NOp::Ones: s << "        const u32 " << L << " = 0u;\n"; break;
                     case NOp::Not: s << "        const u32 " << L << " = " << lref(x.a) << ";\n"; break;
-                    case NOp::And: case NOp::Or: case NOp::Xor:
+                    case NOp::And: case NOp::Or: case NOp::Xor: case NOp::Sel:
                         s << "        const u32 " << L << " = " << lmax(lref(x.a), lref(x.b)) << ";\n";
                         break;
                     case NOp::Shl: case NOp::Shr:
@@ -773,12 +794,28 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     ks.source = s.str();
 }
 
+// The compilation unit of a generated kernel, and its entry point named
+// after the unit's hash (bsz_fk_<16 hex digits>), so profiles tell the
+// kernels of different plans and schedules apart.
+void finish_unit(const PlanOptions& opt, KernelSpec& ks) {
+    const std::string prefix =
+        opt.nontemporal > 0 ? "#define BSZ_NT " + std::to_string(std::min(2, opt.nontemporal)) + "\n" : std::string();
+    const std::string generic = "void bsz_kernel(KArgs A)";
+    const uint64_t h = fnv1a(prefix + kKernelPrelude + ks.source);
+    char nm[40];
+    std::snprintf(nm, sizeof nm, "bsz_fk_%016llx", (unsigned long long)h);
+    ks.entry = nm;
+    const size_t at = ks.source.find(generic);
+    if (at != std::string::npos) ks.source.replace(at, generic.size(), "void " + ks.entry + "(KArgs A)");
+    ks.unit = prefix + std::string(kKernelPrelude) + ks.source;
+}
+
 }  // namespace
 
 std::string SegGraph::key() const {
     std::ostringstream s;
     s << "S" << n_sources << ";";
-    for (const GNode& n : nodes) s << (int)n.op << "," << n.a << "," << n.b << "," << n.k << "," << n.src << ";";
+    for (const GNode& n : nodes) s << (int)n.op << "," << n.a << "," << n.b << "," << n.k << "," << n.src << "," << n.pred << ";";
     s << "|O";
     for (int32_t o : outputs) s << o << ",";
     s << "|P";
@@ -844,6 +881,8 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
         if (!cut) {
             auto inside = [&](int32_t a) { return a >= 0 && kern_of[a] == K; };
             if (needs_external_operand(x) && inside(x.a)) cut = true;
+            // a selection reads its predicate from an earlier kernel's slots
+            if (x.op == NOp::Sel && kern_of[g.pops.at(x.pred)] == K) cut = true;
             if ((int)kernels.back().size() >= opt.max_ops) cut = true;
             int delta = node_last_use[n] > n ? 1 : 0;
             if (arity(x.op) >= 1 && inside(x.a) && node_last_use[x.a] == n) --delta;
@@ -953,6 +992,12 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
             if (arity(x.op) >= 2) ext(x.b);
         }
         for (int32_t p : ks.pops) ext(g.pops[p]);
+        {
+            std::set<int32_t> pr;
+            for (int32_t n : ks.nodes)
+                if (g.nodes[n].op == NOp::Sel) pr.insert(g.nodes[n].pred);
+            ks.preds.assign(pr.begin(), pr.end());
+        }
         for (int32_t a : ins) {
             KInput ki;
             const GNode& x = g.nodes[a];
@@ -971,6 +1016,11 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
     }
     P.out_node_kernel.clear();
     for (int32_t o : g.outputs) P.out_node_kernel.push_back(out_kernel[o]);
+    // every selection's predicate is complete before its kernel starts
+    for (int k = 0; k < NK; ++k)
+        for (int32_t p : P.kernels[k].preds)
+            if (pop_kernel.at(p) >= k)
+                throw std::logic_error("internal: a predicate is produced by the kernel that reads it, or later");
 
     // One compilation unit per kernel (prelude + kernel), so identical kernels
     // of different plans share a cached code object and units compile in
@@ -980,9 +1030,7 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
     for (KernelSpec& ks : P.kernels) {
         gen_kernel(g, opt, ks, emit);
         if (!emit) continue;
-        ks.unit = (opt.nontemporal > 0 ? "#define BSZ_NT " + std::to_string(std::min(2, opt.nontemporal)) + "\n"
-                                        : std::string()) +
-                  std::string(kKernelPrelude) + ks.source;
+        finish_unit(opt, ks);
         mod << "// ---- " << ks.name << " ----\n" << ks.source;
     }
     P.module_source = mod.str();
@@ -1010,9 +1058,7 @@ bool shrink_kernel(PlanSpec& P, size_t k) {
         return false;
     }
     gen_kernel(P.g, o, ks, true);
-    ks.unit = (P.opt.nontemporal > 0 ? "#define BSZ_NT " + std::to_string(std::min(2, P.opt.nontemporal)) + "\n"
-                                     : std::string()) +
-              std::string(kKernelPrelude) + ks.source;
+    finish_unit(P.opt, ks);
     return true;
 }
 

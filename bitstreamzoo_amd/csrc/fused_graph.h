@@ -48,13 +48,18 @@
 
 namespace bsz {
 
-enum class NOp : uint8_t { Input, Zero, Ones, Not, And, Or, Xor, Add, Shl, Shr };
+// Sel: a where predicate `pred` holds, else b, word by word.  A predicate is
+// one of the graph's popcounts read as "not zero"; it is produced by an
+// earlier kernel (its slots are final once that kernel's launch ends) and
+// read by every wave of the consuming kernel before its first row.
+enum class NOp : uint8_t { Input, Zero, Ones, Not, And, Or, Xor, Add, Shl, Shr, Sel };
 
 struct GNode {
     NOp op;
     int32_t a = -1, b = -1;
     uint64_t k = 0;      // shift amount
     int32_t src = -1;    // Input: index of the external source
+    int32_t pred = -1;   // Sel: index into SegGraph::pops of its predicate
 };
 
 struct SegGraph {
@@ -94,6 +99,7 @@ struct KernelSpec {
     std::vector<int32_t> outputs;    // nodes this kernel stores
     std::vector<int32_t> out_slot;   // per output: -1 segment output, else intermediate slot
     std::vector<int32_t> pops;       // index into SegGraph::pops accumulated here
+    std::vector<int32_t> preds;      // index into SegGraph::pops read as predicates (after the inputs in the table)
     int n_state = 0;                 // 64-bit state words
     int peak_live = 0;               // most in-kernel values live at once (partitioner's count)
     int rows_per_step = 1;           // as generated (capped by register budget)
@@ -102,7 +108,8 @@ struct KernelSpec {
     int chunk_rows = 16;             // as generated
     int occupancy = 4;               // waves per SIMD the register budget was set for
     int launch_bounds = 256;         // largest workgroup the kernel accepts
-    std::string source;              // generated HIP (entry point bsz_kernel)
+    std::string source;              // generated HIP
+    std::string entry;               // its entry point: bsz_fk_<hash of the unit>
     std::string unit;                // self-contained compilation unit: prelude + source
 };
 

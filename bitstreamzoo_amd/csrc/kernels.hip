@@ -16,6 +16,8 @@
 #include "device.h"
 
 #include <atomic>
+#include <mutex>
+#include <stdexcept>
 #include <vector>
 
 namespace bsz {
@@ -342,17 +344,68 @@ void launch_shr(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t k, c
     check_launch();
 }
 
-void launch_add(hipStream_t s, uint64_t* dst, const uint64_t* a, const uint64_t* b, const Geom& g, void* scan_state) {
+namespace {
+
+void launch_add_parts(hipStream_t s, uint64_t* dst, const uint64_t* a, const uint64_t* b, const Geom& g,
+                      unsigned* ctl, unsigned* carry, unsigned* unproven) {
     const uint64_t segs = AddScanState::segments(g);
     const uint64_t blocks = (segs + 3) / 4;
     if (blocks > 0x7fffffffull) throw HipError("stream too long for one add launch");
-    unsigned* ctl = reinterpret_cast<unsigned*>(scan_state);
-    unsigned* carry = ctl + 16;
-    unsigned* unproven = carry + segs;
     k_add<<<(unsigned)blocks, 256, 0, s>>>(dst, a, b, g.rows, g.lw, g.lastmask, ctl, carry, unproven, segs);
     check_launch();
     k_add_fix<<<1, 64, 0, s>>>(dst, a, b, g.rows, g.lw, g.lastmask, ctl, carry, unproven, segs);
     check_launch();
+}
+
+// Scratch of the allocation-free addition: 16 control words, the bitmap of
+// unsettled segments and one carry per segment, for streams of up to
+// kNoAllocMaxBits bits (4.3 MB).  Each device holds its own copy for the
+// life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns for that device.
+constexpr uint64_t kNoAllocSegs = kNoAllocMaxBits / 64 / kRowWords / AddScanState::kRows;
+constexpr uint64_t kNoAllocBitmapWords = (kNoAllocSegs + 31) / 32;
+
+std::mutex g_scratch_mu;
+std::vector<unsigned*> g_scratch_ptr;  // per device; null until set up
+
+}  // namespace
+
+static __device__ __align__(256) unsigned scratch[16 + kNoAllocBitmapWords + kNoAllocSegs];
+
+void launch_add(hipStream_t s, uint64_t* dst, const uint64_t* a, const uint64_t* b, const Geom& g, void* scan_state) {
+    unsigned* ctl = reinterpret_cast<unsigned*>(scan_state);
+    unsigned* carry = ctl + 16;
+    launch_add_parts(s, dst, a, b, g, ctl, carry, carry + AddScanState::segments(g));
+}
+
+void setup_noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0) throw HipError("bad device index");
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (g_scratch_ptr[(size_t)device]) return;
+    void* p = nullptr;
+    BSZ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HipError("hipGetSymbolAddress returned no address for the add scratch");
+    g_scratch_ptr[(size_t)device] = static_cast<unsigned*>(p);
+}
+
+void launch_add_noalloc(hipStream_t s, int device, uint64_t* dst, const uint64_t* a, const uint64_t* b,
+                        const Geom& g) {
+    if (g.bitlength <= 0 || (uint64_t)g.bitlength > kNoAllocMaxBits)
+        throw std::invalid_argument("bsz_op_add_noalloc: bitlength must be in [1, 2^34] (the size of its fixed "
+                                    "scratch); use bsz_op_add for longer or unbounded streams");
+    unsigned* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (device >= 0 && (size_t)device < g_scratch_ptr.size()) base = g_scratch_ptr[(size_t)device];
+    }
+    if (!base) throw HipError("bsz_op_add_noalloc: the scratch of this device was not set up");
+    const uint64_t segs = AddScanState::segments(g);
+    // the control words and this geometry's bitmap start from zero (a fill
+    // kernel, not a memset, so the call may be captured into a graph)
+    const uint64_t clear_u32 = 16 + (segs + 31) / 32;
+    launch_zero_u64(s, reinterpret_cast<unsigned long long*>(base), (clear_u32 + 1) / 2);
+    launch_add_parts(s, dst, a, b, g, base, base + 16 + kNoAllocBitmapWords, base + 16);
 }
 
 void launch_popcount(hipStream_t s, unsigned long long* slots, const uint64_t* src, const Geom& g) {

@@ -74,6 +74,14 @@ int bsz_op_xor(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b
 /* op_add (cuda_backend.cuh:69-74 + add_carry.cuh:110-135): one kernel,
  * wave ballot carries + decoupled lookback.  dst must not alias a or b. */
 int bsz_op_add(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bitlength);
+/* op_add with no allocation: the same two kernels as bsz_op_add, with their
+ * scratch (a count, a carry per 256-word segment and a bitmap) in a fixed
+ * 4.3 MB device array of the library instead of memory bsz_op_add allocates
+ * on first use.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under graph capture, serving loops).  bitlength must
+ * be in [1, 2^34]; BSZ_E_INVALID otherwise.  Every call on one device shares
+ * the array, so calls that overlap on a device race. */
+int bsz_op_add_noalloc(bsz_ctx* ctx, uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t bitlength);
 /* op_shl / op_shr (cuda_backend.cuh:75-84).  dst must not alias src. */
 int bsz_op_shl(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, uint64_t k, int64_t bitlength);
 int bsz_op_shr(bsz_ctx* ctx, uint64_t* dst, const uint64_t* src, uint64_t k, int64_t bitlength);
