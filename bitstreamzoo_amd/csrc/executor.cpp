@@ -223,10 +223,11 @@ private:
             if (E.is_nonzero(c)) for (const Stmt& b : s.body) exec(b);
             return;
         }
-        Frame fr = std::move(frames.back());
-        // ops counted directly in this body (nested bodies already moved theirs out)
+        // ops counted directly in this body (nested bodies already moved
+        // theirs out), conditional on every predicate up to this one
         CondOps co;
         co.preds = pred_stack();
+        Frame fr = std::move(frames.back());
         co.ops = ops - fr.ops0;
         for (int i = 0; i < OP_N; ++i) co.mix[i] = mix[i] - fr.mix0[i];
         cond_ops.push_back(std::move(co));
