@@ -394,8 +394,9 @@ int bsz_plan_create(bsz_ctx* ctx, const bsz_program* p, const bsz_case* shape, c
     return guard(ctx, [&] {
         auto h = std::make_unique<bsz_plan>();
         h->ctx = ctx;
-        h->plan = FusedPlan::record(ctx->s, *p->prog, shape->c,
-                                    FusedOptions::from_json(options_json ? options_json : ""), true);
+        const FusedOptions o = FusedOptions::from_json(options_json ? options_json : "");
+        if (o.device_if) throw std::invalid_argument("device_if is an inspection option of bsz_plan_compile_only");
+        h->plan = FusedPlan::record(ctx->s, *p->prog, shape->c, o, true);
         *out = h.release();
     });
 }

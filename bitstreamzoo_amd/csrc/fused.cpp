@@ -79,6 +79,7 @@ FusedOptions FusedOptions::from_json_obj(const Json& j) {
     get("compile", o.compile);
     get("compile_shard", o.compile_shard);
     get("compile_shards", o.compile_shards);
+    get("device_if", o.device_if);
     if (const Json* v = j.get("cache_dir")) o.cache_dir = v->s;
     return o;
 }
@@ -827,7 +828,7 @@ public:
         return p;
     }
 
-    bool supports_predicates() const override { return !recording_; }
+    bool supports_predicates() const override { return !recording_ || fopt_.device_if != 0; }
 
     PopRef predicate(const SV& c) override {
         int32_t n = node_of(c);

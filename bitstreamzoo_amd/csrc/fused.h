@@ -26,6 +26,8 @@ struct FusedOptions {
     int compile = 1;            // compile-only: 0 partitions and generates without running hipRTC
     int compile_shard = 0;      // compile-only: compile the kernels k with k % compile_shards == compile_shard
     int compile_shards = 1;     //   (several processes fill the cache of one plan in parallel)
+    int device_if = 0;          // compile-only: record `if` bodies as device selections (inspection of the plan
+                                //   bsz_run would build; a recorded plan with selections is never run)
     bool verbose = false;
     static FusedOptions from_env();
     // Env defaults overridden by the keys of a flat JSON object

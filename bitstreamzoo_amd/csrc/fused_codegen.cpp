@@ -169,6 +169,10 @@ std::string hex64(uint64_t v) {
     return b;
 }
 
+// State words (carries, advance neighbours) one kernel may carry from row to
+// row; they live in scalar registers, spilled to VGPR lanes when many.
+constexpr int kMaxStateWords = 128;
+
 // Registers a generated kernel may use: the 256 architectural VGPRs at most
 // (kernels that overflowed into AGPRs gave wrong results on gfx950), fewer at
 // a higher requested occupancy.
@@ -847,12 +851,37 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
         if (arity(x.op) >= 1) needed[x.a] = 1;
         if (arity(x.op) >= 2) needed[x.b] = 1;
     }
+    // Partition order.  Without selections it is the recorded order.  With
+    // them, nodes are stably sorted by predicate level (1 + the level of the
+    // popcount a selection reads, the maximum over operands otherwise), so
+    // the selections of independent `if`s share kernels instead of each
+    // forcing a cut after its predicate's kernel.  Still a topological order.
+    std::vector<int32_t> order(N), pos(N);
+    for (int32_t n = 0; n < N; ++n) order[n] = n;
+    {
+        bool has_sel = false;
+        for (int32_t n = 0; n < N; ++n) has_sel |= needed[n] && g.nodes[n].op == NOp::Sel;
+        if (has_sel) {
+            std::vector<int32_t> level(N, 0);
+            for (int32_t n = 0; n < N; ++n) {
+                const GNode& x = g.nodes[n];
+                int32_t l = 0;
+                if (arity(x.op) >= 1) l = std::max(l, level[x.a]);
+                if (arity(x.op) >= 2) l = std::max(l, level[x.b]);
+                if (x.op == NOp::Sel) l = std::max(l, level[g.pops.at(x.pred)] + 1);
+                level[n] = l;
+            }
+            std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return level[a] < level[b]; });
+        }
+    }
+    for (int32_t i = 0; i < N; ++i) pos[order[i]] = i;
+    // last_use in partition positions
     std::vector<int32_t> last_use(N, -1);
     for (int32_t n = 0; n < N; ++n) {
         if (!needed[n]) continue;
         const GNode& x = g.nodes[n];
-        if (arity(x.op) >= 1) last_use[x.a] = std::max(last_use[x.a], n);
-        if (arity(x.op) >= 2) last_use[x.b] = std::max(last_use[x.b], n);
+        if (arity(x.op) >= 1) last_use[x.a] = std::max(last_use[x.a], pos[n]);
+        if (arity(x.op) >= 2) last_use[x.b] = std::max(last_use[x.b], pos[n]);
     }
     // register liveness: outputs and popcounts are stored / counted right
     // after their producer, so only consumer nodes keep a value in registers
@@ -870,9 +899,12 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
     std::vector<std::vector<int32_t>> kernels;
     std::vector<int> peak_live;                    // per kernel: most in-kernel values live at once
     std::set<std::pair<int32_t, uint64_t>> loaded;  // (external operand, shift or 0) words of this kernel
-    int live = 0, n_state = 0;
+    int live = 0, n_state = 0, k_pops = 0;
+    std::vector<int> popped(N, 0);  // popcounts taken of each node
+    for (int32_t p : g.pops) ++popped[p];
     auto is_const = [&](int32_t a) { return g.nodes[a].op == NOp::Zero || g.nodes[a].op == NOp::Ones; };
-    for (int32_t n = 0; n < N; ++n) {
+    for (int32_t i = 0; i < N; ++i) {
+        const int32_t n = order[i];
         if (!needed[n]) continue;
         const GNode& x = g.nodes[n];
         if (is_free(x.op)) continue;
@@ -884,23 +916,27 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
             // a selection reads its predicate from an earlier kernel's slots
             if (x.op == NOp::Sel && kern_of[g.pops.at(x.pred)] == K) cut = true;
             if ((int)kernels.back().size() >= opt.max_ops) cut = true;
-            int delta = node_last_use[n] > n ? 1 : 0;
-            if (arity(x.op) >= 1 && inside(x.a) && node_last_use[x.a] == n) --delta;
-            if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && node_last_use[x.b] == n) --delta;
+            int delta = node_last_use[n] > i ? 1 : 0;
+            if (arity(x.op) >= 1 && inside(x.a) && node_last_use[x.a] == i) --delta;
+            if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && node_last_use[x.b] == i) --delta;
             int new_loads = 0;
             const uint64_t sk = (x.op == NOp::Shl || x.op == NOp::Shr) ? (x.op == NOp::Shr ? (1ull << 63) : 0) | x.k : 0;
             if (arity(x.op) >= 1 && !inside(x.a) && !is_const(x.a) && !loaded.count({x.a, sk})) ++new_loads;
             if (arity(x.op) >= 2 && !inside(x.b) && !is_const(x.b) && x.b != x.a && !loaded.count({x.b, 0})) ++new_loads;
-            if (live + delta + (int)loaded.size() + new_loads > value_budget) cut = true;
-            if (n_state >= 512 && (x.op == NOp::Add || (stateful_shift(x) && inside(x.a)))) cut = true;
+            // a popcount keeps a 32-bit counter per lane: half a value
+            if (live + delta + (int)loaded.size() + new_loads + (k_pops + popped[n] + 1) / 2 > value_budget)
+                cut = true;
+            if (n_state >= kMaxStateWords && (x.op == NOp::Add || (stateful_shift(x) && inside(x.a)))) cut = true;
         }
         if (cut) {
             kernels.emplace_back();
             peak_live.push_back(0);
             live = 0;
             n_state = 0;
+            k_pops = 0;
             loaded.clear();
         }
+        k_pops += popped[n];
         const int KK = (int)kernels.size() - 1;
         auto inside = [&](int32_t a) { return a >= 0 && kern_of[a] == KK; };
         {
@@ -908,9 +944,9 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
             if (arity(x.op) >= 1 && !inside(x.a) && !is_const(x.a)) loaded.insert({x.a, sk});
             if (arity(x.op) >= 2 && !inside(x.b) && !is_const(x.b)) loaded.insert({x.b, 0});
         }
-        if (arity(x.op) >= 1 && inside(x.a) && node_last_use[x.a] == n) --live;
-        if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && node_last_use[x.b] == n) --live;
-        if (node_last_use[n] > n) ++live;
+        if (arity(x.op) >= 1 && inside(x.a) && node_last_use[x.a] == i) --live;
+        if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && node_last_use[x.b] == i) --live;
+        if (node_last_use[n] > i) ++live;
         peak_live.back() = std::max(peak_live.back(), live);
         if (x.op == NOp::Add || (stateful_shift(x) && inside(x.a))) ++n_state;
         kern_of[n] = KK;
