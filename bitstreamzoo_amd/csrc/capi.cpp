@@ -392,6 +392,7 @@ static int64_t put_str(const std::string& j, char* buf, uint64_t cap) {
 int bsz_plan_create(bsz_ctx* ctx, const bsz_program* p, const bsz_case* shape, const char* options_json,
                     bsz_plan** out) {
     return guard(ctx, [&] {
+        if (!ctx || !p || !shape || !out) throw std::invalid_argument("bsz_plan_create: null argument");
         auto h = std::make_unique<bsz_plan>();
         h->ctx = ctx;
         const FusedOptions o = FusedOptions::from_json(options_json ? options_json : "");

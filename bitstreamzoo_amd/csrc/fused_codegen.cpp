@@ -171,7 +171,7 @@ std::string hex64(uint64_t v) {
 
 // State words (carries, advance neighbours) one kernel may carry from row to
 // row; they live in scalar registers, spilled to VGPR lanes when many.
-constexpr int kMaxStateWords = 128;
+constexpr int kMaxStateWords = 32;
 
 // Registers a generated kernel may use: the 256 architectural VGPRs at most
 // (kernels that overflowed into AGPRs gave wrong results on gfx950), fewer at

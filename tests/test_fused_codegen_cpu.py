@@ -224,3 +224,30 @@ def test_code_object_cache_rejects_foreign_files(tmp_path):
     info = api.compile_plan_only(prog, case, opts)
     assert info["kernels"][0]["vgprs"] > 0
     assert open(files[0], "rb").read() == good  # recompiled and rewritten
+
+
+def test_device_if_plan_groups_selections_by_level():
+    """A 200-pattern regex-shaped program (417 `if`s) recorded with device
+    selections (the plan bsz_run builds for it) partitions by predicate
+    level: far fewer kernels than `if`s, each selection reading a predicate
+    finished by an earlier kernel (checked inside the planner), and no
+    kernel over the state-word cap.  Such a recorded plan is never run
+    through bsz_plan_create."""
+    import regex_ifs
+    src = regex_ifs.program(200, seed=7)
+    case = regex_ifs.text_case(2000, seed=6, alphabet=6)
+    info = api.compile_plan_only(api.Program(src), api.Case.from_bscase(case), {"device_if": 1, "compile": 0})
+    ks = info["kernels"]
+    assert len(ks) < src.count("if (") / 4, len(ks)
+    assert max(k["state_words"] for k in ks) <= 32
+
+
+def test_plan_create_rejects_null_arguments():
+    """bsz_plan_create checks its arguments before touching a device."""
+    import ctypes as C
+    L = api.lib()
+    h = C.c_void_p()
+    w = workloads.matchstar(256)
+    rc = L.bsz_plan_create(None, api.Program(w.source).h, api.Case.from_bscase(w.cases[0]).h, b'{"device_if":1}',
+                           C.byref(h))
+    assert rc != 0
