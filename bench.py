@@ -190,7 +190,8 @@ def bench_matchstar(ctx, args):
         if not np.array_equal(got, e) or p.read_int(0, "n") != words_popcount(e):
             raise SystemExit("matchstar output differs from the host computation")
     ms = time_plans(ctx, B.plans, args.steps, args.warmup, api.FUSED)
-    log(f"matchstar: fused {ms:.4f} ms/step")
+    unproven = sum(p.unproven(0) for p in B.plans)
+    log(f"matchstar: fused {ms:.4f} ms/step ({unproven} chunks left unproven by their warm-up)")
     perop = time_plans(ctx, B.plans, max(2, args.steps // 4), 1, api.PEROP)
     log(f"matchstar: per-op {perop:.4f} ms/step")
     info = B.plans[0].info()
@@ -198,7 +199,8 @@ def bench_matchstar(ctx, args):
     for r in residents:
         r.free()
     return ws[0], {"ms_per_step": ms, "perop_ms_per_step": perop, "bit_ops": ws[0].bit_ops,
-                   "alg_bytes": ws[0].alg_bytes, "plan": info, "input_sets": rot, "options": opts, "tune": tune}
+                   "alg_bytes": ws[0].alg_bytes, "plan": info, "input_sets": rot, "options": opts, "tune": tune,
+                   "unproven_chunks": unproven}
 
 
 def bench_shift_and(ctx, args):
@@ -304,7 +306,7 @@ def bench_learned(ctx, args):
     log(f"learned: {model.members} networks trained on {len(ms)} autotuner measurements in "
         f"{model.train_seconds:.2f} s (loss {loss:.4f})")
     cands = schedsearch.sample_space(64, 12345)
-    held = schedsearch.random_specs(args.learned_programs, 7, nodes=(2000, 5000), log2_bytes=(23, 25))
+    held = schedsearch.random_specs(args.learned_programs, 7, nodes=(1000, 2000), log2_bytes=(21, 22))
     rows = schedsearch.evaluate(ctx, model, held, cands, processes=min(16, os.cpu_count() or 1), log=log)
     return {"train_rows": int(len(ms)), "train_s": round(model.train_seconds, 3), "candidates": len(cands),
             "programs": [f"{r.nodes} nodes x {r.n_bytes} B" for r in rows],
@@ -421,7 +423,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner; use fixed default schedules")
     ap.add_argument("--schedule", default="", help="JSON options for the measured workload's plan (skips tuning)")
-    ap.add_argument("--learned-programs", type=int, default=2,
+    ap.add_argument("--learned-programs", type=int, default=1,
                     help="held-out programs for the learned-vs-autotuned comparison (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -522,6 +524,8 @@ def main():
                     "perop_ms_per_step": round(rr["perop_ms_per_step"], 5), "schedule": rr["options"],
                     "tuning_s": rr["tune"]["tuning_s"] if rr.get("tune") else None,
                     "speedup_fused_over_perop": round(rr["perop_ms_per_step"] / m, 2)}
+        if "unproven_chunks" in rr:
+            ex[name]["unproven_chunks"] = rr["unproven_chunks"]
     if ex:
         line["other_configs"] = ex
     if learned is not None:

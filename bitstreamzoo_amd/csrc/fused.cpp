@@ -274,7 +274,9 @@ public:
         if (todo.empty()) return;
         auto t0 = std::chrono::steady_clock::now();
         const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-        const size_t nthr = std::min<size_t>(todo.size(), std::min(16u, hw));
+        // BSZ_COMPILE_THREADS caps the hipRTC threads of this process
+        const unsigned cap = (unsigned)std::max(1, env_int("BSZ_COMPILE_THREADS", 16));
+        const size_t nthr = std::min<size_t>(todo.size(), std::min(cap, hw));
         std::atomic<size_t> next{0};
         std::mutex err_mu;
         std::string err;

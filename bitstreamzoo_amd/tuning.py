@@ -42,6 +42,10 @@ def _shape_from_desc(desc) -> api.Case:
 
 
 def _compile_worker(job):
+    # each worker process compiles one unit at a time: the processes are the
+    # parallelism (many processes times many hipRTC threads exhaust memory)
+    import os
+    os.environ["BSZ_COMPILE_THREADS"] = "1"
     source, desc, options = job
     api.compile_plan_only(api.Program(source), _shape_from_desc(desc), options)
     return True
@@ -70,6 +74,22 @@ def precompile(source: str, cases: list[BsCase], options_list: list[dict], proce
         f.result()
         if log and ((i + 1) % every == 0 or i + 1 == len(jobs)):
             log(f"compiled {i + 1}/{len(jobs)} shards ({time.perf_counter() - t0:.0f} s)")
+    return time.perf_counter() - t0
+
+
+def precompile_many(jobs: list, processes: int = 16, log=None) -> float:
+    """Fill the code-object cache for many programs at once: `jobs` is a list
+    of (source, cases, options); each is one compile job (all of one plan's
+    kernels) on the shared worker pool.  Returns the wall seconds spent."""
+    import concurrent.futures as cf
+    t0 = time.perf_counter()
+    futs = [_pool(processes).submit(_compile_worker, (src, _shape_desc(cases), dict(opts)))
+            for src, cases, opts in jobs]
+    every = max(1, len(futs) // 10)
+    for i, f in enumerate(cf.as_completed(futs)):
+        f.result()
+        if log and ((i + 1) % every == 0 or i + 1 == len(futs)):
+            log(f"compiled {i + 1}/{len(futs)} programs ({time.perf_counter() - t0:.0f} s)")
     return time.perf_counter() - t0
 
 
