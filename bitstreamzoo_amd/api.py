@@ -277,7 +277,7 @@ def compile_plan_only(program: "Program", shape: "Case", options: dict | None = 
     opts = json.dumps(options or {}).encode()
     L = lib()
     return json.loads(_fill_str(lambda buf, cap: L.bsz_plan_compile_only(program.h, shape.h, opts, buf, cap),
-                                cap=1 << 22))
+                                cap=1 << 25))
 
 
 def describe_plans(program: "Program", shape: "Case", options_list: list[dict]) -> list[dict]:

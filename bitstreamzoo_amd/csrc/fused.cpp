@@ -1240,7 +1240,9 @@ std::unique_ptr<FusedPlan> FusedPlan::record(hipStream_t s, const Program& prog,
         }
         I.ints.push_back(x);
     }
-    I.spec = std::make_shared<PlanSpec>(build_plan(g, to_plan_options(opt)));
+    // kernel sources only when they will be compiled (partition and kernel
+    // shapes are all a description needs)
+    I.spec = std::make_shared<PlanSpec>(build_plan(g, to_plan_options(opt), compile));
     if (compile && !I.spec->kernels.empty()) {
         I.rec->jit().compile_all(*I.spec, &I.rec->compile_ms(), std::max(0, opt.compile_shard),
                                  std::max(1, opt.compile_shards));

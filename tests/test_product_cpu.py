@@ -210,3 +210,26 @@ def test_static_ops_pinned_to_reference_on_control_flow_programs():
         assert ref == ora == prod, (seed, ref, ora, prod, src)
         seen_control += ("if (" in src) or ("while" in src)
     assert seen_control >= 20
+
+
+def test_frontend_and_planner_at_a_million_ops():
+    """A generated 1M-node regex-style DAG (1.25M DSL ops, 40 MB of source,
+    the size class of the reference's regex_large) goes through the
+    front-end, the op counter, the recording walk and the partitioner and
+    slot planner in bounded time; kernel sources are not generated for a
+    description."""
+    import time
+    from bitstreamzoo_amd import dag_gen, tuning
+    p = dag_gen.generate(1_000_000, seed=4)
+    assert p.n_dsl_ops > 1_200_000
+    case = dag_gen.text_case(1 << 12, seed=11)
+    t0 = time.perf_counter()
+    prog = api.Program(p.source)
+    shape = tuning.shape_case([case])
+    assert prog.static_ops(shape) == p.n_dsl_ops
+    t_parse = time.perf_counter() - t0
+    (d,) = api.describe_plans(prog, shape, [{"warmup_rows": 4}])
+    t_plan = time.perf_counter() - t0 - t_parse
+    assert len(d["kernels"]) > 10_000 and 0 < d["slots"] < 10_000
+    assert sum(k["nodes"] for k in d["kernels"]) >= p.n_dsl_ops * 0.9
+    assert t_parse < 20 and t_plan < 60, (t_parse, t_plan)
