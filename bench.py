@@ -192,15 +192,19 @@ def bench_matchstar(ctx, args):
     ms = time_plans(ctx, B.plans, args.steps, args.warmup, api.FUSED)
     unproven = sum(p.unproven(0) for p in B.plans)
     log(f"matchstar: fused {ms:.4f} ms/step ({unproven} chunks left unproven by their warm-up)")
-    perop = time_plans(ctx, B.plans, max(2, args.steps // 4), 1, api.PEROP)
-    log(f"matchstar: per-op {perop:.4f} ms/step")
+    perop = None if args.no_perop else time_plans(ctx, B.plans, max(2, args.steps // 4), 1, api.PEROP)
+    if perop is not None:
+        log(f"matchstar: per-op {perop:.4f} ms/step")
     info = B.plans[0].info()
     B.free()
     for r in residents:
         r.free()
-    return ws[0], {"ms_per_step": ms, "perop_ms_per_step": perop, "bit_ops": ws[0].bit_ops,
-                   "alg_bytes": ws[0].alg_bytes, "plan": info, "input_sets": rot, "options": opts, "tune": tune,
-                   "unproven_chunks": unproven}
+    out = {"ms_per_step": ms, "bit_ops": ws[0].bit_ops,
+           "alg_bytes": ws[0].alg_bytes, "plan": info, "input_sets": rot, "options": opts, "tune": tune,
+           "unproven_chunks": unproven}
+    if perop is not None:
+        out["perop_ms_per_step"] = perop
+    return ws[0], out
 
 
 def bench_shift_and(ctx, args):
@@ -226,13 +230,18 @@ def bench_shift_and(ctx, args):
     ms = time_plans(ctx, [plan], args.steps, args.warmup, api.FUSED)
     log(f"shift_and: fused {ms:.4f} ms/step; timing the per-op executor (eager launches)")
     ops_per_inst = plan.info()["graph_nodes"]
-    perop = time_plans(ctx, [plan], 2, 1, api.PEROP, launches_per_run=n * ops_per_inst)
-    log(f"shift_and: per-op {perop:.4f} ms/step")
+    perop = None
+    if not args.no_perop:
+        perop = time_plans(ctx, [plan], 2, 1, api.PEROP, launches_per_run=n * ops_per_inst)
+        log(f"shift_and: per-op {perop:.4f} ms/step")
     info = plan.info()
     B.free()
     res.free()
-    return w, {"ms_per_step": ms, "perop_ms_per_step": perop, "bit_ops": w.bit_ops, "alg_bytes": w.alg_bytes,
-               "plan": info, "instances": n, "checked_instances": len(check_idx), "options": opts, "tune": tune}
+    out = {"ms_per_step": ms, "bit_ops": w.bit_ops, "alg_bytes": w.alg_bytes,
+           "plan": info, "instances": n, "checked_instances": len(check_idx), "options": opts, "tune": tune}
+    if perop is not None:
+        out["perop_ms_per_step"] = perop
+    return w, out
 
 
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9   # CUs x 64 lanes x 2.4 GHz: 32-bit VALU lane-ops per second
@@ -521,9 +530,10 @@ def main():
         ex[name] = {"ms_per_step": round(m, 5), "value": round(rr["bit_ops"] / (m * 1e-3) / 1e9, 1),
                     "GBps": round(rr["alg_bytes"] / (m * 1e-3) / 1e9, 1),
                     "hbm_frac": round(rr["alg_bytes"] / (m * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-                    "perop_ms_per_step": round(rr["perop_ms_per_step"], 5), "schedule": rr["options"],
-                    "tuning_s": rr["tune"]["tuning_s"] if rr.get("tune") else None,
-                    "speedup_fused_over_perop": round(rr["perop_ms_per_step"] / m, 2)}
+                    "schedule": rr["options"], "tuning_s": rr["tune"]["tuning_s"] if rr.get("tune") else None}
+        if "perop_ms_per_step" in rr:
+            ex[name]["perop_ms_per_step"] = round(rr["perop_ms_per_step"], 5)
+            ex[name]["speedup_fused_over_perop"] = round(rr["perop_ms_per_step"] / m, 2)
         if "unproven_chunks" in rr:
             ex[name]["unproven_chunks"] = rr["unproven_chunks"]
     if ex:
