@@ -1,6 +1,11 @@
 // bsdata_io.cpp — JSON parsing and the .bsdata reader (see bsdata_io.h).
 #include "bsdata_io.h"
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -333,15 +338,45 @@ std::string words_to_hex(const uint64_t* w, size_t n) {
     return out;
 }
 
+namespace {
+
+// A read-only memory map of a whole file (the reference reads the file into
+// a vector and copies each stream out again, bsdata_reader.h:262-299; here
+// streams are decoded straight from the page cache).
+struct MappedFile {
+    const uint8_t* p = nullptr;
+    size_t n = 0;
+    int fd = -1;
+    explicit MappedFile(const std::string& path) {
+        fd = ::open(path.c_str(), O_RDONLY);
+        if (fd < 0) throw std::runtime_error("Cannot open bsdata file: " + path);
+        struct stat st;
+        if (::fstat(fd, &st) != 0) { ::close(fd); throw std::runtime_error("Cannot stat bsdata file: " + path); }
+        n = (size_t)st.st_size;
+        if (n) {
+            void* m = ::mmap(nullptr, n, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (m == MAP_FAILED) { ::close(fd); throw std::runtime_error("Cannot map bsdata file: " + path); }
+            (void)::madvise(m, n, MADV_SEQUENTIAL);
+            p = static_cast<const uint8_t*>(m);
+        }
+    }
+    ~MappedFile() {
+        if (p) ::munmap(const_cast<uint8_t*>(p), n);
+        if (fd >= 0) ::close(fd);
+    }
+    MappedFile(const MappedFile&) = delete;
+    MappedFile& operator=(const MappedFile&) = delete;
+};
+
+}  // namespace
+
 std::vector<CaseData> read_bsdata_file(const std::string& path) {
-    std::ifstream f(path, std::ios::binary);
-    if (!f) throw std::runtime_error("Cannot open bsdata file: " + path);
-    std::vector<uint8_t> raw((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-    auto sep = std::find(raw.begin(), raw.end(), (uint8_t)0);
-    if (sep == raw.end()) throw std::runtime_error("Invalid bsdata file (no NUL separator): " + path);
-    size_t sp = (size_t)(sep - raw.begin());
-    Json header = json_parse(std::string(raw.begin(), raw.begin() + sp));
-    Blob bin{raw.data() + sp + 1, raw.size() - sp - 1};
+    MappedFile f(path);
+    const uint8_t* sep = f.n ? static_cast<const uint8_t*>(std::memchr(f.p, 0, f.n)) : nullptr;
+    if (!sep) throw std::runtime_error("Invalid bsdata file (no NUL separator): " + path);
+    const size_t sp = (size_t)(sep - f.p);
+    Json header = json_parse(std::string(reinterpret_cast<const char*>(f.p), sp));
+    Blob bin{f.p + sp + 1, f.n - sp - 1};
     std::vector<CaseData> out;
     const Json* multi = header.get("cases");
     if (!multi) multi = header.get("layers");

@@ -4,6 +4,7 @@
 //   bsim_hip program.bs [--trace] [--word-bits=32|64] [--reuse-mem] [--warmup]
 //                       [--input file.bsdata ...] [--dump-input]
 //                       [--executor=fused|perop] [--device=N]
+//                       [--output-bsdata out.bsdata]
 //
 // Without --input one JSON case is read from stdin (keys inputs, params,
 // input_arrays, bitlength / n_vectors, iterations, feedback).  stdout carries
@@ -11,6 +12,10 @@
 // several.  Exit 1 with "Parse error: ...", "Error reading bsdata: ..." or
 // "Error: ..." on stderr.  --reuse-mem and --trace are accepted for
 // compatibility (memory is always planned; there is no trace output).
+// --output-bsdata writes the output streams in binary to a .bsdata-style
+// file (the JSON printed on stdout, a NUL, then the words) and prints that
+// JSON with {"_bin": [offset, bytes]} in place of each hex string: the hex
+// of large outputs (2.2 GB for 33 streams of 2^28 bits) is never built.
 #include "bsdata_io.h"
 #include "executor.h"
 #include "fused.h"
@@ -37,9 +42,12 @@ int main(int argc, char** argv) {
     int word_bits = 64, device = 0;
     std::string executor = "fused";
     std::vector<std::string> inputs;
+    std::string out_bsdata;
     for (int i = 2; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--trace" || a == "--reuse-mem") continue;
+        if (a == "--output-bsdata" && i + 1 < argc) { out_bsdata = argv[++i]; continue; }
+        if (a.rfind("--output-bsdata=", 0) == 0) { out_bsdata = a.substr(16); continue; }
         if (a == "--warmup") warmup = true;
         else if (a.rfind("--word-bits=", 0) == 0) word_bits = std::atoi(a.c_str() + 12);
         else if (a == "--input" && i + 1 < argc) inputs.push_back(argv[++i]);
@@ -115,6 +123,7 @@ int main(int argc, char** argv) {
             executor == "perop" ? make_perop_engine(s) : make_fused_engine(s, FusedOptions::from_env());
         const bool batch = cases.size() > 1;
         std::string out = batch ? "[" : "";
+        std::vector<uint8_t> blob;
         for (size_t i = 0; i < cases.size(); ++i) {
             if (warmup) {
                 CaseData w = cases[i];
@@ -123,7 +132,7 @@ int main(int argc, char** argv) {
             }
             RunOutcome o = run_case_iterations(*prog, cases[i], *eng, parse_ms);
             if (word_bits == 32) o.n_words = o.bitlength > 0 ? (uint64_t)(o.bitlength + 31) / 32 : o.n_words * 2;
-            std::string j = result_json(o);
+            std::string j = result_json(o, out_bsdata.empty() ? nullptr : &blob);
             if (batch) {
                 j.pop_back();
                 j += ",\"name\":" + json_escape(cases[i].name) + "}";
@@ -133,6 +142,14 @@ int main(int argc, char** argv) {
             }
         }
         if (batch) out += "]";
+        if (!out_bsdata.empty()) {
+            std::ofstream of(out_bsdata, std::ios::binary);
+            if (!of) throw std::runtime_error("cannot write " + out_bsdata);
+            of.write(out.data(), (std::streamsize)out.size());
+            of.put('\0');
+            of.write(reinterpret_cast<const char*>(blob.data()), (std::streamsize)blob.size());
+            if (!of) throw std::runtime_error("write failed: " + out_bsdata);
+        }
         std::cout << out << "\n";
         eng.reset();
         (void)hipStreamDestroy(s);

@@ -430,6 +430,29 @@ private:
     }
 };
 
+// A plan whose kernels all compile within the architectural VGPRs.  Under
+// automatic occupancy a partition that is still refused after every kernel
+// was shrunk is partitioned again with half the fusion budget (max_live,
+// max_ops), up to three times; an explicit occupancy is the caller's choice
+// and its refusal stands.  Any partition gives the same bits.
+std::shared_ptr<PlanSpec> clean_plan(Jit& jit, const SegGraph& g, PlanOptions o, double* compile_ms,
+                                     std::vector<KernelRegs>* regs_out) {
+    for (int attempt = 0;; ++attempt) {
+        auto spec = std::make_shared<PlanSpec>(build_plan(g, o));
+        if (spec->kernels.empty()) return spec;
+        try {
+            std::vector<KernelRegs> r = jit.compile_clean(*spec, compile_ms);
+            if (regs_out) *regs_out = std::move(r);
+            return spec;
+        } catch (const RunError& e) {
+            if (o.occupancy > 0 || attempt >= 3 || std::string(e.what()).find("schedule refused") == std::string::npos)
+                throw;
+            o.max_live = std::max(16, o.max_live / 2);
+            o.max_ops = std::max(16, o.max_ops / 2);
+        }
+    }
+}
+
 // ════════════════════════════════════════════════════════════════════════
 // Plan runtime: device tables and state for one plan over a set of instances
 // ════════════════════════════════════════════════════════════════════════
@@ -1019,7 +1042,7 @@ public:
         auto it = plans_.find(key);
         if (it != plans_.end()) return it->second;
         CachedPlan cp;
-        cp.spec = std::make_shared<PlanSpec>(build_plan(g, popt_));
+        cp.spec = clean_plan(jit_, g, popt_, &compile_ms_, nullptr);
         if (!cp.spec->kernels.empty()) cp.mod = jit_.load(*cp.spec, &compile_ms_);
         cp.rt = std::make_unique<PlanRT>();
         cp.rt->spec = cp.spec;
@@ -1242,13 +1265,15 @@ std::unique_ptr<FusedPlan> FusedPlan::record(hipStream_t s, const Program& prog,
     }
     // kernel sources only when they will be compiled (partition and kernel
     // shapes are all a description needs)
-    I.spec = std::make_shared<PlanSpec>(build_plan(g, to_plan_options(opt), compile));
-    if (compile && !I.spec->kernels.empty()) {
-        I.rec->jit().compile_all(*I.spec, &I.rec->compile_ms(), std::max(0, opt.compile_shard),
-                                 std::max(1, opt.compile_shards));
-        // a shard job (one of several processes filling the cache) does not
-        // report registers: its plan's other kernels may not exist yet
-        if (std::max(1, opt.compile_shards) == 1) I.compiled_regs = I.rec->jit().compile_clean(*I.spec, &I.rec->compile_ms());
+    if (compile && std::max(1, opt.compile_shards) == 1) {
+        I.spec = clean_plan(I.rec->jit(), g, to_plan_options(opt), &I.rec->compile_ms(), &I.compiled_regs);
+    } else {
+        I.spec = std::make_shared<PlanSpec>(build_plan(g, to_plan_options(opt), compile));
+        // a shard job (one of several processes filling the cache) compiles
+        // its share and reports no registers: other kernels may not exist yet
+        if (compile && !I.spec->kernels.empty())
+            I.rec->jit().compile_all(*I.spec, &I.rec->compile_ms(), std::max(0, opt.compile_shard),
+                                     std::max(1, opt.compile_shards));
     }
     return fp;
 }

@@ -172,6 +172,8 @@ std::string hex64(uint64_t v) {
 // State words (carries, advance neighbours) one kernel may carry from row to
 // row; they live in scalar registers, spilled to VGPR lanes when many.
 constexpr int kMaxStateWords = 32;
+// Kernels of at most this many nodes prove their chunks exact in the warm-up.
+constexpr int kTrackNodes = 128;
 
 // Registers a generated kernel may use: the 256 architectural VGPRs at most
 // (kernels that overflowed into AGPRs gave wrong results on gfx950), fewer at
@@ -196,6 +198,9 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     }
     ks.n_state = n_state;
     const bool has_state = n_state > 0;
+    // Lane-exactness tracking in warm-up rows costs a scalar per node; large
+    // kernels skip it (their chunks are all checked by the fix-up launch).
+    const bool track = has_state && (int)ks.nodes.size() <= kTrackNodes;
 
     // which graph nodes need a plain value of an input (loaded per row)
     std::set<int32_t> plain_inputs;
@@ -634,7 +639,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                     s << "        const u64 " << v << " = s" << t << " + (((u" << t << " ^ G" << t << " ^ P" << t
                       << ") >> lane) & 1ull);\n";
                     s << "        " << st << " = (u64)((t" << t << " < G" << t << ") | (u" << t << " < t" << t << "));\n";
-                    if (warm) {
+                    if (warm && track) {
                         // lanes that do not propagate a carry (they generate
                         // or kill it) at or above the damaged range end it
                         const std::string xs = "xs" + std::to_string(state_of.at(n));
@@ -659,7 +664,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                         s << "        const u64 " << v << " = fsl(" << ref(x.a) << ", prev_word(" << ref(x.a) << ", " << st
                           << "), " << x.k << "u);\n";
                         s << "        " << st << " = lane63(" << ref(x.a) << ");\n";
-                        if (warm) {
+                        if (warm && track) {
                             // lane 0 takes the state; lane i > 0 takes operand lanes i and i-1
                             const std::string xs = "xs" + std::to_string(state_of.at(n)), la = lref(x.a);
                             s << "        const u32 " << L << " = " << la << " == 0u ? (" << xs << " ? 0u : 1u) : (" << la
@@ -675,7 +680,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                     break;
                 case NOp::Input: break;
             }
-            if (warm) {
+            if (warm && track) {
                 switch (x.op) {
                     case NOp::Zero: case NOp::Ones: s << "        const u32 " << L << " = 0u;\n"; break;
                     case NOp::Not: s << "        const u32 " << L << " = " << lref(x.a) << ";\n"; break;
@@ -715,13 +720,17 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         // whose warm-up proves every state exact is final; the others are
         // marked for the fix-up launch.
         s << "    if (warm) {\n";
-        for (auto& kv : state_of) s << "      u32 xs" << kv.second << " = 0u;\n";
+        if (track)
+            for (auto& kv : state_of) s << "      u32 xs" << kv.second << " = 0u;\n";
         s << "      for (u64 r = r_begin - warm; r < r_begin; r += " << U << "ull) {\n";
         emit_loads(U, "r_begin");
         for (int u = 0; u < U; ++u) emit_compute(u, true);
         s << "      }\n";
-        s << "      u32 proven = 1u;\n";
-        for (auto& kv : state_of) s << "      proven &= xs" << kv.second << ";\n";
+        // a kernel too large to track its lanes leaves every such chunk to
+        // the fix-up launch's comparison
+        s << "      u32 proven = " << (track ? "1u" : "0u") << ";\n";
+        if (track)
+            for (auto& kv : state_of) s << "      proven &= xs" << kv.second << ";\n";
         s << "      if (!proven && lane == 0) {\n";
         s << "        __hip_atomic_fetch_or(&A.unproven[c >> 5], 1u << (c & 31u), __ATOMIC_RELAXED, AGENT);\n";
         s << "        __hip_atomic_fetch_add(&A.kstat[2], 1u, __ATOMIC_RELAXED, AGENT);\n";

@@ -3,6 +3,7 @@
 
 #include <chrono>
 #include <cstdio>
+#include <cstring>
 
 namespace bsz {
 
@@ -55,7 +56,18 @@ static std::string num(double v) {
     return b;
 }
 
-std::string result_json(const RunOutcome& o) {
+namespace {
+
+std::string bin_ref(std::vector<uint8_t>* blob, const std::vector<uint64_t>& w) {
+    const size_t off = blob->size(), nb = w.size() * 8;
+    blob->resize(off + nb);
+    if (nb) std::memcpy(blob->data() + off, w.data(), nb);
+    return "{\"_bin\":[" + std::to_string(off) + "," + std::to_string(nb) + "]}";
+}
+
+}  // namespace
+
+std::string result_json(const RunOutcome& o, std::vector<uint8_t>* blob) {
     std::string s = "{\"dynamic_ops\":" + std::to_string(o.dynamic_ops);
     s += ",\"engine\":" + o.engine_stats;
     s += ",\"exec_ms\":" + num(o.exec_ms);
@@ -76,7 +88,8 @@ std::string result_json(const RunOutcome& o) {
         auto it = o.res.streams.find(n);
         if (it != o.res.streams.end()) {
             key(n);
-            s += "\"" + words_to_hex(it->second.data(), it->second.size()) + "\"";
+            if (blob) s += bin_ref(blob, it->second);
+            else s += "\"" + words_to_hex(it->second.data(), it->second.size()) + "\"";
         }
         auto ai = o.res.arrays.find(n);
         if (ai != o.res.arrays.end()) {
@@ -84,8 +97,9 @@ std::string result_json(const RunOutcome& o) {
             s += "{";
             bool f2 = true;
             for (auto& e : ai->second) {
-                s += (f2 ? "\"" : ",\"") + std::to_string(e.first) + "\":\"" +
-                     words_to_hex(e.second.data(), e.second.size()) + "\"";
+                s += (f2 ? "\"" : ",\"") + std::to_string(e.first) + "\":";
+                if (blob) s += bin_ref(blob, e.second);
+                else s += "\"" + words_to_hex(e.second.data(), e.second.size()) + "\"";
                 f2 = false;
             }
             s += "}";

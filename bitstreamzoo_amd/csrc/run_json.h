@@ -23,6 +23,9 @@ struct RunOutcome {
 };
 
 RunOutcome run_case_iterations(const Program& prog, const CaseData& c, Engine& eng, double parse_ms);
-std::string result_json(const RunOutcome& o);
+// With `blob`, every output stream's words (little-endian, masked to the
+// bitlength) are appended to it and the JSON holds {"_bin": [offset, bytes]}
+// in place of the hex string (the .bsdata encoding of a stream).
+std::string result_json(const RunOutcome& o, std::vector<uint8_t>* blob = nullptr);
 
 }  // namespace bsz

@@ -87,16 +87,46 @@ def parse_outputs(outputs: dict) -> dict:
     return res
 
 
+def read_output_bsdata(path: str) -> list[dict]:
+    """The per-case JSON objects of a `bsim_hip --output-bsdata` file with
+    every {"_bin": [offset, bytes]} output replaced by its words as a Python
+    int (the hex strings of the default output, without building them)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    sep = raw.index(b"\0")
+    head = json.loads(raw[:sep].decode())
+    blob = memoryview(raw)[sep + 1:]
+
+    def val(v):
+        if isinstance(v, dict) and "_bin" in v:
+            off, nb = v["_bin"]
+            return int.from_bytes(blob[off:off + nb], "little")
+        return v
+
+    cases = head if isinstance(head, list) else [head]
+    for c in cases:
+        outs = c.get("outputs", {})
+        for k, v in list(outs.items()):
+            if isinstance(v, dict) and "_bin" not in v:
+                outs[k] = {int(i): val(x) for i, x in v.items()}
+            else:
+                outs[k] = val(v)
+    return cases
+
+
 class HipBackend:
-    """Runs programs through bsim_hip (the MI355X executor)."""
+    """Runs programs through bsim_hip (the MI355X executor).  With
+    binary_outputs, output streams come back through a binary .bsdata file
+    (--output-bsdata) instead of hex strings in the JSON."""
 
     def __init__(self, bsim_path: str | None = None, word_bits: int = 64, reuse_mem: bool = True,
-                 executor: str = "fused", timeout: int = 7200):
+                 executor: str = "fused", timeout: int = 7200, binary_outputs: bool = False):
         self.bsim_path = bsim_path or CLI_PATH
         self.word_bits = word_bits
         self.reuse_mem = reuse_mem
         self.executor = executor
         self.timeout = timeout
+        self.binary_outputs = binary_outputs
         if not os.path.exists(self.bsim_path):
             raise FileNotFoundError(f"bsim_hip not found at {self.bsim_path}; build it with "
                                     "`python -m bitstreamzoo_amd.build`")
@@ -119,6 +149,19 @@ class HipBackend:
             bsdata_path: str | None = None) -> tuple[dict, int, float]:
         """(outputs as Python ints, dynamic op count, exec_ms) of one case."""
         cmd = self._cmd(prog)
+        tmp = None
+        if self.binary_outputs:
+            import tempfile
+            fd, tmp = tempfile.mkstemp(suffix=".bsdata")
+            os.close(fd)
+            cmd += ["--output-bsdata", tmp]
+        try:
+            return self._run(cmd, tmp, inputs, params, input_arrays, bitlength, iterations, feedback, bsdata_path)
+        finally:
+            if tmp:
+                os.unlink(tmp)
+
+    def _run(self, cmd, tmp, inputs, params, input_arrays, bitlength, iterations, feedback, bsdata_path):
         if bsdata_path:
             proc = subprocess.run(cmd + ["--input", bsdata_path], capture_output=True, text=True,
                                   timeout=self.timeout, env=subprocess_env())
@@ -127,6 +170,9 @@ class HipBackend:
             proc = subprocess.run(cmd, input=json.dumps(case), capture_output=True, text=True,
                                   timeout=self.timeout, env=subprocess_env())
         self._check(proc)
+        if tmp:
+            out = read_output_bsdata(tmp)[0]
+            return out["outputs"], out["op_count"], out.get("exec_ms", 0.0)
         out = json.loads(proc.stdout)
         if isinstance(out, list):  # a .bsdata file with several cases: the first one
             out = out[0]

@@ -79,3 +79,30 @@ def test_run_batch_one_process_per_file(tmp_path):
     for r, (exp, n) in zip(res, exps):
         o = harness.parse_outputs(r["outputs"])
         assert o["result"] == exp and o["n"] == n
+
+
+def test_read_output_bsdata_decodes_bin_references(tmp_path):
+    """The --output-bsdata file: the JSON bsim_hip prints, a NUL, then the
+    words; streams and array elements referenced by offset and length."""
+    blob = (5).to_bytes(8, "little") + (1 << 63).to_bytes(8, "little") + (7).to_bytes(16, "little")
+    head = {"op_count": 3, "outputs": {"s": {"_bin": [0, 16]}, "a": {"2": {"_bin": [16, 16]}}, "n": 4}}
+    p = tmp_path / "o.bsdata"
+    p.write_bytes(json.dumps(head).encode() + b"\0" + blob)
+    (c,) = harness.read_output_bsdata(str(p))
+    assert c["outputs"] == {"s": 5 | (1 << 127), "a": {2: 7}, "n": 4} and c["op_count"] == 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("executor", ["fused", "perop"])
+def test_binary_outputs_equal_hex_outputs(tmp_path, executor):
+    """bsim_hip --output-bsdata returns the same outputs as the hex JSON."""
+    bs = tmp_path / "matchstar.bs"
+    bs.write_text(workloads.MATCHSTAR_BS)
+    prog = harness.ProgramRef(str(bs))
+    c = workloads.matchstar(300_001, seed=9).cases[0]
+    args = ({"M": words_to_int(c.inputs["M"]), "C": words_to_int(c.inputs["C"])},)
+    hexo = harness.HipBackend(executor=executor).run(prog, *args, bitlength=c.bitlength)
+    bino = harness.HipBackend(executor=executor, binary_outputs=True).run(prog, *args, bitlength=c.bitlength)
+    assert hexo[0] == bino[0] and hexo[1] == bino[1]
+    exp, n = _matchstar_expected(c)
+    assert bino[0]["result"] == exp and bino[0]["n"] == n
