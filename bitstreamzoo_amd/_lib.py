@@ -55,6 +55,7 @@ _SIGS = {
     "bsz_case_set_array_elem": (C.c_int, [vp, C.c_char_p, C.c_int64, u64p, C.c_uint64]),
     "bsz_case_set_broadcast": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_int64]),
     "bsz_case_set_iterations": (C.c_int, [vp, C.c_int]),
+    "bsz_case_set_padding": (C.c_int, [vp, C.c_int]),
     "bsz_case_set_feedback": (C.c_int, [vp, C.c_char_p, C.c_char_p]),
     "bsz_case_read_bsdata": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(vp)), C.POINTER(C.c_int)]),
     "bsz_case_array_free": (C.c_int, [C.POINTER(vp), C.c_int]),

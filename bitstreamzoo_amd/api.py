@@ -178,10 +178,21 @@ class Case:
             lib().bsz_case_free(self.h)
             self.h = None
 
+    def set_padding(self, word_bits: int) -> "Case":
+        """Value semantics of the bits at or above the bitlength: 0 keeps
+        them clear (DSL.md:273-275); 64 or 32 gives them the values bsim_cuda
+        gives them with --word-bits=64|32 (its streams are padded to a
+        multiple of 8 words, cuda_backend.cuh:23-25; ~, ONES, + and << leave
+        the padding bits set and >> brings them back)."""
+        check(lib().bsz_case_set_padding(self.h, int(word_bits)))
+        return self
+
     @classmethod
-    def from_bscase(cls, c: BsCase) -> "Case":
+    def from_bscase(cls, c: BsCase, padding: int = 0) -> "Case":
         k = cls(c.bitlength)
         L = lib()
+        if padding:
+            k.set_padding(padding)
         for n, v in c.params.items():
             L.bsz_case_set_param(k.h, n.encode(), int(v))
         for n, w in c.inputs.items():

@@ -45,6 +45,10 @@ struct CaseData {
     std::map<std::string, std::vector<bool>> broadcast;
     int iterations = 1;
     std::map<std::string, std::string> feedback;
+    // 0: clean values (no bit at or above the bitlength, DSL.md:273-275);
+    // 64 or 32: the padding bits of bsim_cuda's padded streams with words of
+    // that width, which ~, ONES, + and << leave set and >> brings back.
+    int bsim_padding = 0;
 };
 
 // All cases of a .bsdata file (read_bsdata_multi, bsdata_reader.h:262-299).

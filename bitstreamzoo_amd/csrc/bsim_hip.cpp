@@ -4,7 +4,7 @@
 //   bsim_hip program.bs [--trace] [--word-bits=32|64] [--reuse-mem] [--warmup]
 //                       [--input file.bsdata ...] [--dump-input]
 //                       [--executor=fused|perop] [--device=N]
-//                       [--output-bsdata out.bsdata]
+//                       [--output-bsdata out.bsdata] [--padding=clean|bsim]
 //
 // Without --input one JSON case is read from stdin (keys inputs, params,
 // input_arrays, bitlength / n_vectors, iterations, feedback).  stdout carries
@@ -12,6 +12,10 @@
 // several.  Exit 1 with "Parse error: ...", "Error reading bsdata: ..." or
 // "Error: ..." on stderr.  --reuse-mem and --trace are accepted for
 // compatibility (memory is always planned; there is no trace output).
+// --padding=bsim gives bits at or above the bitlength the values bsim_cuda
+// gives them with the chosen --word-bits (its streams are padded to a
+// multiple of 8 words; ~, ONES, + and << leave padding bits set and >> can
+// bring them back); the default, clean, keeps them zero as DSL.md specifies.
 // --output-bsdata writes the output streams in binary to a .bsdata-style
 // file (the JSON printed on stdout, a NUL, then the words) and prints that
 // JSON with {"_bin": [offset, bytes]} in place of each hex string: the hex
@@ -42,12 +46,13 @@ int main(int argc, char** argv) {
     int word_bits = 64, device = 0;
     std::string executor = "fused";
     std::vector<std::string> inputs;
-    std::string out_bsdata;
+    std::string out_bsdata, padding = "clean";
     for (int i = 2; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--trace" || a == "--reuse-mem") continue;
         if (a == "--output-bsdata" && i + 1 < argc) { out_bsdata = argv[++i]; continue; }
         if (a.rfind("--output-bsdata=", 0) == 0) { out_bsdata = a.substr(16); continue; }
+        if (a.rfind("--padding=", 0) == 0) { padding = a.substr(10); continue; }
         if (a == "--warmup") warmup = true;
         else if (a.rfind("--word-bits=", 0) == 0) word_bits = std::atoi(a.c_str() + 12);
         else if (a == "--input" && i + 1 < argc) inputs.push_back(argv[++i]);
@@ -57,6 +62,10 @@ int main(int argc, char** argv) {
     }
     if (word_bits != 32 && word_bits != 64) {
         std::cerr << "Error: --word-bits must be 32 or 64\n";
+        return 1;
+    }
+    if (padding != "clean" && padding != "bsim") {
+        std::cerr << "Error: --padding must be clean or bsim\n";
         return 1;
     }
     if (executor != "fused" && executor != "perop") {
@@ -104,6 +113,9 @@ int main(int argc, char** argv) {
         }
         return 0;
     }
+
+    if (padding == "bsim")
+        for (CaseData& c : cases) c.bsim_padding = word_bits;
 
     auto t0 = std::chrono::steady_clock::now();
     std::unique_ptr<Program> prog;

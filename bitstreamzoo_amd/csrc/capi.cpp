@@ -174,8 +174,7 @@ int bsz_h2d(bsz_ctx* ctx, uint64_t* dev, const uint64_t* host, uint64_t n, int64
 
 int bsz_d2h(bsz_ctx* ctx, uint64_t* host, const uint64_t* dev, uint64_t n) {
     return guard(ctx, [&] {
-        BSZ_HIP(hipMemcpyAsync(host, dev, n * 8, hipMemcpyDeviceToHost, ctx->s));
-        BSZ_HIP(hipStreamSynchronize(ctx->s));
+        stage_d2h(ctx->s, host, dev, n * 8);
     });
 }
 
@@ -288,6 +287,12 @@ int bsz_case_set_broadcast(bsz_case* c, const char* n, const uint8_t* bits, int6
     return BSZ_OK;
 }
 int bsz_case_set_iterations(bsz_case* c, int it) { c->c.iterations = it; return BSZ_OK; }
+int bsz_case_set_padding(bsz_case* c, int word_bits) {
+    if (!c || (word_bits != 0 && word_bits != 32 && word_bits != 64))
+        return fail(nullptr, BSZ_E_INVALID, "bsz_case_set_padding: word_bits must be 0, 32 or 64");
+    c->c.bsim_padding = word_bits;
+    return BSZ_OK;
+}
 int bsz_case_set_feedback(bsz_case* c, const char* o, const char* i) { c->c.feedback[o] = i; return BSZ_OK; }
 
 int bsz_case_read_bsdata(const char* path, bsz_case*** cases_out, int* n_out) {

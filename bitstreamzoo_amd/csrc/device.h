@@ -38,6 +38,14 @@ struct Geom {
     uint64_t rows = 1;       // 64-word rows
     uint64_t lastmask = ~0ull;  // valid bits of word lw-1
 
+    // Words an op result may occupy, and the mask of the last of them.  By
+    // default (clean values, DSL.md:273-275) they are lw and lastmask: no bit
+    // at or above the bitlength is ever set.  bsim_padding() widens them to
+    // the reference executor's padded stream, whose padding bits ~, ONES, +
+    // and << leave set and >> can bring back.
+    uint64_t opw = 1;
+    uint64_t opmask = ~0ull;
+
     static Geom make(int64_t bitlength, uint64_t words_if_unbounded) {
         Geom g;
         g.bitlength = bitlength;
@@ -45,6 +53,22 @@ struct Geom {
         g.rows = (g.lw + kRowWords - 1) / kRowWords;
         uint64_t rem = bitlength > 0 ? (uint64_t)bitlength % 64 : 0;
         g.lastmask = rem ? ((1ull << rem) - 1) : ~0ull;
+        g.opw = g.lw;
+        g.opmask = g.lastmask;
+        return g;
+    }
+    // The padded stream of bsim_cuda with `word_bits`-bit words: its word
+    // count rounded up to a multiple of 8 (cuda_backend.cuh:23-25), i.e.
+    // dirty bits up to a multiple of 512 (64-bit) or 256 (32-bit) bits.
+    // word_bits 0 keeps clean values.
+    Geom bsim_padding(int word_bits) const {
+        Geom g = *this;
+        if (word_bits != 32 && word_bits != 64) return g;
+        const uint64_t bits = bitlength > 0 ? (uint64_t)bitlength : lw * 64;
+        const uint64_t words = word_bits == 32 && bitlength > 0 ? (bits + 31) / 32 : (word_bits == 32 ? lw * 2 : lw);
+        const uint64_t padded = (words + 7) & ~uint64_t(7);
+        g.opw = std::min<uint64_t>(alloc_words(), word_bits == 32 ? padded / 2 : padded);
+        g.opmask = ~0ull;
         return g;
     }
     uint64_t alloc_words() const { return rows * kRowWords; }
@@ -121,8 +145,15 @@ uint64_t read_popcount_slots(hipStream_t s, const unsigned long long* slots);
 void launch_nonzero(hipStream_t s, unsigned long long* slot, const uint64_t* src, const Geom& g);
 void launch_zero_u64(hipStream_t s, unsigned long long* p, uint64_t n);
 // Load `n` host-side words into a row-padded buffer, truncated to the stream
-// (bytes_to_words, word_utils.h:91-108) and masked to the bitlength.
+// (bytes_to_words, word_utils.h:91-108): masked to the bitlength for clean
+// values, kept to whole words under bsim padding, as bsim_cuda's reader does.
 void upload_stream(hipStream_t s, uint64_t* dst, const uint64_t* host, uint64_t n, const Geom& g);
+// Host-to-device copy, asynchronous on s; copies of 4 MB and more pass through
+// two pinned staging buffers of 32 MB (BSZ_PINNED_STAGING=0: direct copy).
+// The host buffer may be reused as soon as this returns.
+void stage_h2d(hipStream_t s, void* dst, const void* src, size_t bytes);
+// Device-to-host copy through the same buffers; synchronous.
+void stage_d2h(hipStream_t s, void* dst, const void* src, size_t bytes);
 // Before a fused plan launch: for each of n_ctl control blocks of 4 words
 // (ticket, done, epoch, mismatches) clear ticket/done/mismatches and advance
 // the epoch; clear n_pop popcount accumulators.

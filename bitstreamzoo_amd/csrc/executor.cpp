@@ -60,7 +60,7 @@ private:
     void walk() {
         uint64_t max_words = 1;
         for (auto& kv : C.inputs) max_words = std::max<uint64_t>(max_words, kv.second.size());
-        g = Geom::make(C.bitlength, max_words);
+        g = Geom::make(C.bitlength, max_words).bsim_padding(C.bsim_padding);
         E.begin(g);
         zero_c = E.constant(false);
         ones_c = E.constant(true);
@@ -80,6 +80,7 @@ private:
             if (vars[o] && !absent_vars.count(o)) {
                 std::vector<uint64_t> w(g.lw);
                 E.download(vars[o], w.data());
+                if (g.bitlength > 0) w[g.lw - 1] &= g.lastmask;  // padding bits are never output
                 r.streams[nm] = std::move(w);
             }
             auto ai = arrays.find(o);
@@ -89,6 +90,7 @@ private:
                     if (absent_elems.count({o, e.first})) continue;
                     std::vector<uint64_t> w(g.lw);
                     E.download(e.second, w.data());
+                    if (g.bitlength > 0) w[g.lw - 1] &= g.lastmask;
                     dst[e.first] = std::move(w);
                 }
             }
@@ -612,8 +614,7 @@ public:
     }
 
     void download(const SV& a, uint64_t* host) override {
-        BSZ_HIP(hipMemcpyAsync(host, a->buf->ptr, g_.lw * 8, hipMemcpyDeviceToHost, s_));
-        BSZ_HIP(hipStreamSynchronize(s_));
+        stage_d2h(s_, host, a->buf->ptr, g_.lw * 8);
     }
 
     void flush() override {}

@@ -519,6 +519,8 @@ public:
             d.rows = geoms[i].rows;
             d.lw = geoms[i].lw;
             d.lastmask = geoms[i].lastmask;
+            d.opw = geoms[i].opw;
+            d.opmask = geoms[i].opmask;
             d.chunk_begin = (uint32_t)nc;
             d.n_chunks = (uint32_t)((d.rows + chunk_rows - 1) / chunk_rows);
             nc += d.n_chunks;
@@ -910,8 +912,7 @@ public:
             if (a->konst && g_.lw) host[g_.lw - 1] &= g_.lastmask;
             return;
         }
-        BSZ_HIP(hipMemcpyAsync(host, a->buf->ptr, g_.lw * 8, hipMemcpyDeviceToHost, s_));
-        BSZ_HIP(hipStreamSynchronize(s_));
+        stage_d2h(s_, host, a->buf->ptr, g_.lw * 8);
     }
 
     void flush() override;

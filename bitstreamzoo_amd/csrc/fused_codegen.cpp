@@ -39,7 +39,7 @@ uint64_t fnv1a(const std::string& s) {
 const char* kKernelPrelude = R"HIP(
 typedef unsigned long long u64;
 typedef unsigned int u32;
-struct InstDesc { u64 rows; u64 lw; u64 lastmask; u32 chunk_begin; u32 n_chunks; };
+struct InstDesc { u64 rows; u64 lw; u64 lastmask; u64 opw; u64 opmask; u32 chunk_begin; u32 n_chunks; };
 struct KArgs {
   const u64* const* in; u64* const* out; u64* pop_acc;
   const InstDesc* inst; const u32* chunk_inst;
@@ -347,7 +347,11 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         for (int p = 0; p < NPOP; ++p) pop_of[g.pops[ks.pops[p]]].push_back(p);
         for (int u = 0; u < U; ++u) {
             s << "    if (ok" << u << ") {\n";
-            s << "      const u64x2 m = {word_mask(w" << u << ", D.lw, D.lastmask), word_mask(w" << u << " + 1, D.lw, D.lastmask)};\n";
+            // m: the words an op result may occupy; mb: the bitlength (popcounts)
+            s << "      const u64x2 m = {word_mask(w" << u << ", D.opw, D.opmask), word_mask(w" << u << " + 1, D.opw, D.opmask)};\n";
+            if (NPOP)
+                s << "      const u64x2 mb = {word_mask(w" << u << ", D.lw, D.lastmask), word_mask(w" << u
+                  << " + 1, D.lw, D.lastmask)};\n";
             auto ref = [&](int32_t a) -> std::string {
                 const GNode& x = g.nodes[a];
                 if (in_kernel.count(a)) return "v" + std::to_string(a);
@@ -365,7 +369,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                         s << "        if (lv" << u << ") lane_st16(urow_w(out" << j << " + ru" << u << " * 64ull), off16, t);\n";
                 if (pi != pop_of.end())
                     for (int p : pi->second)
-                        s << "        pc" << p << " += (u32)(__builtin_popcountll(t.x) + __builtin_popcountll(t.y));\n";
+                        s << "        pc" << p << " += (u32)(__builtin_popcountll(t.x & mb.x) + __builtin_popcountll(t.y & mb.y));\n";
                 s << "      }\n";
                 if (opt.store_order) s << "      __builtin_amdgcn_sched_barrier(0);\n";
             };
@@ -569,7 +573,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                 auto word = [&](long long d) -> std::string {
                     if (src.op == NOp::Zero) return "0ull";
                     std::string wd = "w" + std::to_string(u) + (d < 0 ? " - " + std::to_string(-d) + "ll" : " + " + std::to_string(d) + "ll");
-                    if (src.op == NOp::Ones) return "word_mask(" + wd + ", D.lw, D.lastmask)";
+                    if (src.op == NOp::Ones) return "word_mask(" + wd + ", D.opw, D.opmask)";
                     if (d == 0) return "x" + std::to_string(x.a) + "_" + std::to_string(u);
                     return "z" + std::to_string(x.a) + "_" + (d < 0 ? "m" : "p") + std::to_string(std::llabs(d)) + "_" + std::to_string(u);
                 };
@@ -589,7 +593,9 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     // a lane that neither propagates nor lies in the damaged range is exact.
     auto emit_compute = [&](int u, bool warm) {
         s << "      if (ok" << u << ") {\n";
-        s << "        const u64 m = word_mask(w" << u << ", D.lw, D.lastmask);\n";
+        // m: the words an op result may occupy; mb: the bitlength (popcounts)
+        s << "        const u64 m = word_mask(w" << u << ", D.opw, D.opmask);\n";
+        if (NPOP && !warm) s << "        const u64 mb = word_mask(w" << u << ", D.lw, D.lastmask);\n";
         auto ref = [&](int32_t a) -> std::string {
             const GNode& x = g.nodes[a];
             if (in_kernel.count(a)) return "v" + std::to_string(a);
@@ -614,7 +620,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                 for (int j : oi->second)
                     s << "        lane_st(urow_w(out" << j << " + ru" << u << " * 64ull), off8, " << ref(n) << " & m);\n";
             if (pi != pop_of.end())
-                for (int p : pi->second) s << "        pc" << p << " += (u32)__builtin_popcountll(" << ref(n) << " & m);\n";
+                for (int p : pi->second) s << "        pc" << p << " += (u32)__builtin_popcountll(" << ref(n) << " & mb);\n";
             // without it the scheduler may sink every store to the end of the
             // step and keep all the outputs of a row live at once
             if (opt.store_order) s << "        __builtin_amdgcn_sched_barrier(0);\n";

@@ -138,6 +138,8 @@ struct InstDesc {
     uint64_t rows;
     uint64_t lw;
     uint64_t lastmask;
+    uint64_t opw;        // words an op result may occupy (Geom::opw)
+    uint64_t opmask;     // mask of word opw-1
     uint32_t chunk_begin;
     uint32_t n_chunks;
 };

@@ -16,6 +16,8 @@
 #include "device.h"
 
 #include <atomic>
+#include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <stdexcept>
 #include <vector>
@@ -313,18 +315,18 @@ void launch_pointwise(hipStream_t s, PwOp op, uint64_t* dst, const uint64_t* a, 
     const uint64_t pairs = g.alloc_words() / 2;
     const unsigned grid = blocks_for(pairs, 256);
     switch (op) {
-        case PwOp::And: k_pointwise<(int)PwOp::And><<<grid, 256, 0, s>>>(dst, a, b, pairs, g.lw, g.lastmask); break;
-        case PwOp::Or:  k_pointwise<(int)PwOp::Or><<<grid, 256, 0, s>>>(dst, a, b, pairs, g.lw, g.lastmask); break;
-        case PwOp::Xor: k_pointwise<(int)PwOp::Xor><<<grid, 256, 0, s>>>(dst, a, b, pairs, g.lw, g.lastmask); break;
-        case PwOp::Not: k_pointwise<(int)PwOp::Not><<<grid, 256, 0, s>>>(dst, a, b, pairs, g.lw, g.lastmask); break;
-        case PwOp::Copy: k_pointwise<(int)PwOp::Copy><<<grid, 256, 0, s>>>(dst, a, b, pairs, g.lw, g.lastmask); break;
+        case PwOp::And: k_pointwise<(int)PwOp::And><<<grid, 256, 0, s>>>(dst, a, b, pairs, g.opw, g.opmask); break;
+        case PwOp::Or:  k_pointwise<(int)PwOp::Or><<<grid, 256, 0, s>>>(dst, a, b, pairs, g.opw, g.opmask); break;
+        case PwOp::Xor: k_pointwise<(int)PwOp::Xor><<<grid, 256, 0, s>>>(dst, a, b, pairs, g.opw, g.opmask); break;
+        case PwOp::Not: k_pointwise<(int)PwOp::Not><<<grid, 256, 0, s>>>(dst, a, b, pairs, g.opw, g.opmask); break;
+        case PwOp::Copy: k_pointwise<(int)PwOp::Copy><<<grid, 256, 0, s>>>(dst, a, b, pairs, g.opw, g.opmask); break;
     }
     check_launch();
 }
 
 void launch_fill(hipStream_t s, uint64_t* dst, bool ones, const Geom& g) {
     const uint64_t pairs = g.alloc_words() / 2;
-    k_fill<<<blocks_for(pairs, 256), 256, 0, s>>>(dst, ones ? ~0ull : 0ull, pairs, g.lw, g.lastmask);
+    k_fill<<<blocks_for(pairs, 256), 256, 0, s>>>(dst, ones ? ~0ull : 0ull, pairs, g.opw, g.opmask);
     check_launch();
 }
 
@@ -332,7 +334,7 @@ void launch_shl(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t k, c
     const uint64_t n = g.alloc_words();
     const uint64_t q = k / 64;
     if (q >= n) { launch_fill(s, dst, false, g); return; }
-    k_shl<<<blocks_for(n, 256), 256, 0, s>>>(dst, src, q, (unsigned)(k % 64), n, g.lw, g.lastmask);
+    k_shl<<<blocks_for(n, 256), 256, 0, s>>>(dst, src, q, (unsigned)(k % 64), n, g.opw, g.opmask);
     check_launch();
 }
 
@@ -351,9 +353,9 @@ void launch_add_parts(hipStream_t s, uint64_t* dst, const uint64_t* a, const uin
     const uint64_t segs = AddScanState::segments(g);
     const uint64_t blocks = (segs + 3) / 4;
     if (blocks > 0x7fffffffull) throw HipError("stream too long for one add launch");
-    k_add<<<(unsigned)blocks, 256, 0, s>>>(dst, a, b, g.rows, g.lw, g.lastmask, ctl, carry, unproven, segs);
+    k_add<<<(unsigned)blocks, 256, 0, s>>>(dst, a, b, g.rows, g.opw, g.opmask, ctl, carry, unproven, segs);
     check_launch();
-    k_add_fix<<<1, 64, 0, s>>>(dst, a, b, g.rows, g.lw, g.lastmask, ctl, carry, unproven, segs);
+    k_add_fix<<<1, 64, 0, s>>>(dst, a, b, g.rows, g.opw, g.opmask, ctl, carry, unproven, segs);
     check_launch();
 }
 
@@ -438,11 +440,97 @@ void launch_zero_u64(hipStream_t s, unsigned long long* p, uint64_t n) {
     check_launch();
 }
 
+namespace {
+
+// Two pinned host buffers that large host-to-device copies pass through: the
+// copy of the next piece into one buffer overlaps the DMA of the other, and
+// the DMA engine reads page-locked memory at full PCIe rate.
+struct PinnedStage {
+    static constexpr size_t kBytes = size_t(32) << 20;
+    std::mutex mu;
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    int device = -1;
+};
+PinnedStage g_stage;
+
+// Buffers and events for the current device (events belong to a device);
+// called with the mutex held.
+void stage_ready() {
+    int dev = 0;
+    BSZ_HIP(hipGetDevice(&dev));
+    if (g_stage.device == dev) return;
+    for (int b = 0; b < 2; ++b) {
+        if (g_stage.done[b]) { (void)hipEventSynchronize(g_stage.done[b]); (void)hipEventDestroy(g_stage.done[b]); }
+        g_stage.done[b] = nullptr;
+        if (!g_stage.buf[b]) BSZ_HIP(hipHostMalloc(&g_stage.buf[b], PinnedStage::kBytes, hipHostMallocPortable));
+        BSZ_HIP(hipEventCreateWithFlags(&g_stage.done[b], hipEventDisableTiming));
+    }
+    g_stage.device = dev;
+}
+
+bool staging_enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("BSZ_PINNED_STAGING");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
+}  // namespace
+
+void stage_h2d(hipStream_t s, void* dst, const void* src, size_t bytes) {
+    if (bytes < (size_t(4) << 20) || !staging_enabled()) {
+        BSZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_stage.mu);
+    stage_ready();
+    const char* from = static_cast<const char*>(src);
+    char* to = static_cast<char*>(dst);
+    for (size_t off = 0, i = 0; off < bytes; off += PinnedStage::kBytes, ++i) {
+        const int b = (int)(i & 1);
+        const size_t n = std::min(PinnedStage::kBytes, bytes - off);
+        BSZ_HIP(hipEventSynchronize(g_stage.done[b]));  // the buffer's previous transfer has finished
+        std::memcpy(g_stage.buf[b], from + off, n);
+        BSZ_HIP(hipMemcpyAsync(to + off, g_stage.buf[b], n, hipMemcpyHostToDevice, s));
+        BSZ_HIP(hipEventRecord(g_stage.done[b], s));
+    }
+}
+
+void stage_d2h(hipStream_t s, void* dst, const void* src, size_t bytes) {
+    if (bytes < (size_t(4) << 20) || !staging_enabled()) {
+        BSZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        BSZ_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_stage.mu);
+    stage_ready();
+    for (int b = 0; b < 2; ++b) BSZ_HIP(hipEventSynchronize(g_stage.done[b]));  // earlier uploads have left
+    const char* from = static_cast<const char*>(src);
+    char* to = static_cast<char*>(dst);
+    // piece i goes to buffer i & 1; piece i - 1 is copied out while piece i transfers
+    const size_t pieces = (bytes + PinnedStage::kBytes - 1) / PinnedStage::kBytes;
+    for (size_t i = 0; i <= pieces; ++i) {
+        if (i < pieces) {
+            const size_t off = i * PinnedStage::kBytes, n = std::min(PinnedStage::kBytes, bytes - off);
+            BSZ_HIP(hipMemcpyAsync(g_stage.buf[i & 1], from + off, n, hipMemcpyDeviceToHost, s));
+            BSZ_HIP(hipEventRecord(g_stage.done[i & 1], s));
+        }
+        if (i > 0) {
+            const size_t j = i - 1, off = j * PinnedStage::kBytes, n = std::min(PinnedStage::kBytes, bytes - off);
+            BSZ_HIP(hipEventSynchronize(g_stage.done[j & 1]));
+            std::memcpy(to + off, g_stage.buf[j & 1], n);
+        }
+    }
+}
+
 void upload_stream(hipStream_t s, uint64_t* dst, const uint64_t* host, uint64_t n, const Geom& g) {
     const uint64_t k = std::min<uint64_t>(n, g.lw);
-    if (k) BSZ_HIP(hipMemcpyAsync(dst, host, k * 8, hipMemcpyHostToDevice, s));
+    if (k) stage_h2d(s, dst, host, k * 8);
     const uint64_t total = g.alloc_words();
-    k_tail<<<std::max(1u, blocks_for(total - k, 256)), 256, 0, s>>>(dst, k, total, g.lw, g.lastmask);
+    // opmask is lastmask for clean values and all ones under bsim padding
+    k_tail<<<std::max(1u, blocks_for(total - k, 256)), 256, 0, s>>>(dst, k, total, g.lw, g.opmask);
     check_launch();
 }
 

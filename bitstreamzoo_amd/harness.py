@@ -117,12 +117,16 @@ def read_output_bsdata(path: str) -> list[dict]:
 class HipBackend:
     """Runs programs through bsim_hip (the MI355X executor).  With
     binary_outputs, output streams come back through a binary .bsdata file
-    (--output-bsdata) instead of hex strings in the JSON."""
+    (--output-bsdata) instead of hex strings in the JSON.  padding="bsim"
+    gives bits above the bitlength bsim_cuda's values for word_bits
+    (--padding=bsim); the default keeps them clear."""
 
     def __init__(self, bsim_path: str | None = None, word_bits: int = 64, reuse_mem: bool = True,
-                 executor: str = "fused", timeout: int = 7200, binary_outputs: bool = False):
+                 executor: str = "fused", timeout: int = 7200, binary_outputs: bool = False,
+                 padding: str = "clean"):
         self.bsim_path = bsim_path or CLI_PATH
         self.word_bits = word_bits
+        self.padding = padding
         self.reuse_mem = reuse_mem
         self.executor = executor
         self.timeout = timeout
@@ -135,6 +139,8 @@ class HipBackend:
         cmd = [self.bsim_path, prog.source_path, f"--word-bits={self.word_bits}", f"--executor={self.executor}"]
         if self.reuse_mem:
             cmd.append("--reuse-mem")
+        if self.padding != "clean":
+            cmd.append(f"--padding={self.padding}")
         return cmd
 
     def _check(self, proc: subprocess.CompletedProcess):

@@ -106,6 +106,13 @@ int bsz_case_set_array_elem(bsz_case* c, const char* name, int64_t idx, const ui
 /* count elements, one bit each (1 = ONES), LSB first (bsdata.py:96-102). */
 int bsz_case_set_broadcast(bsz_case* c, const char* name, const uint8_t* bits, int64_t count);
 int bsz_case_set_iterations(bsz_case* c, int iterations);
+/* Value semantics of bits at or above the bitlength.  0 (default): clean,
+ * as DSL.md:273-275 and the reference's Python simulator.  64 or 32: those of
+ * bsim_cuda with --word-bits=64|32, whose streams are padded to a multiple of
+ * 8 words (cuda_backend.cuh:23-25, common.cuh:35-45): ~, ONES, + and << leave
+ * the padding bits set and >> brings them back into range.  Outputs,
+ * popcounts and if/while conditions are bounded by the bitlength either way. */
+int bsz_case_set_padding(bsz_case* c, int word_bits);
 int bsz_case_set_feedback(bsz_case* c, const char* out_name, const char* in_name);
 /* All cases of a .bsdata file (read_bsdata_multi, bsdata_reader.h:262-299). */
 int bsz_case_read_bsdata(const char* path, bsz_case*** cases_out, int* n_out);

@@ -656,7 +656,16 @@ struct Machine {
     }
 
     // Clean semantics: nothing at or above bitlength (DSL.md:273-275).
+    // BSIM32: bsim_cuda with 32-bit words pads each stream to a multiple of 8
+    // such words (cuda_backend.cuh:23-25), 256 bits, so the dirty region ends
+    // there rather than at the 512-bit multiple of the 64-bit layout.
     void clean(Buf& b) const {
+        if (sem == BSZO_SEM_BSIM32) {
+            uint64_t w32 = bitlength > 0 ? ((uint64_t)bitlength + 31) / 32 : 2 * lw;
+            uint64_t keep = ((w32 + 7) & ~uint64_t(7)) / 2;
+            if (nw > keep) std::memset(b.p + keep, 0, (nw - keep) * 8);
+            return;
+        }
         if (sem != BSZO_SEM_CLEAN || bitlength <= 0) return;
         uint64_t rem = (uint64_t)bitlength % 64;
         if (rem) b.p[lw - 1] &= (1ull << rem) - 1;
@@ -971,12 +980,15 @@ int bszo_run(bszo_case* c, int variant, char* err, size_t errlen) {
 
         auto t0 = std::chrono::steady_clock::now();
         // Input loading (interpreter.inl:288-312); inputs are truncated to the
-        // stream length (bsdata_reader.h:54-58, bytes_to_words).
+        // stream length (bsdata_reader.h:54-58, bytes_to_words): to the
+        // bitlength in clean semantics, to whole 64-bit words in bsim's, whose
+        // reader keeps bits above the bitlength in the last word.
+        const uint64_t in_words = c->sem == BSZO_SEM_CLEAN ? m->nw : m->lw;
         for (const std::string& name : c->prog.inputs) {
             auto it = c->inputs.find(name);
             if (it != c->inputs.end()) {
                 Buf& b = m->var(name);
-                uint64_t k = std::min<uint64_t>(it->second.size(), m->nw);
+                uint64_t k = std::min<uint64_t>(it->second.size(), in_words);
                 std::memcpy(b.p, it->second.data(), k * 8);
                 if (k < m->nw) std::memset(b.p + k, 0, (m->nw - k) * 8);
                 m->clean(b);
@@ -989,7 +1001,7 @@ int bszo_run(bszo_case* c, int variant, char* err, size_t errlen) {
                         auto& arr = m->arrays[name];
                         auto slot = arr.find(e.first);
                         if (slot == arr.end()) slot = arr.emplace(e.first, Buf(m->nw)).first;
-                        uint64_t k = std::min<uint64_t>(e.second.size(), m->nw);
+                        uint64_t k = std::min<uint64_t>(e.second.size(), in_words);
                         std::memcpy(slot->second.p, e.second.data(), k * 8);
                         if (k < m->nw) std::memset(slot->second.p + k, 0, (m->nw - k) * 8);
                         m->clean(slot->second);

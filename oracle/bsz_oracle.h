@@ -22,6 +22,8 @@
  *   BSZO_SEM_BSIM: cpu_backend.h:90-96 — ONES/NOT/+/<< leave padding bits
  *     dirty and only the bounded reductions and the final output mask hide
  *     them.  The two differ only when `>>` pulls padding into [0, bitlength).
+ *   BSZO_SEM_BSIM32: the same with bsim_cuda's --word-bits=32 layout, whose
+ *     padding ends at a multiple of 256 bits instead of 512.
  *
  * Parity status of each function is listed in the header comment of
  * bsz_oracle.cpp.
@@ -38,7 +40,7 @@ extern "C" {
 
 typedef struct bszo_case bszo_case;
 
-enum { BSZO_SEM_CLEAN = 0, BSZO_SEM_BSIM = 1 };
+enum { BSZO_SEM_CLEAN = 0, BSZO_SEM_BSIM = 1, BSZO_SEM_BSIM32 = 2 };
 
 /* Execution variants of the word kernels (all give identical bits).
  *   SCALAR      plain loops, ripple add — the reference model
