@@ -1,7 +1,8 @@
 // fused.cpp — the fused executor: segment recording, plan cache, hipRTC
-// compilation with an on-disk code-object cache, plan launch (one prep
-// kernel + one persistent-queue kernel per partition, optionally captured
-// into a hipGraph) and the batched plan interface.
+// compilation with an on-disk code-object cache, plan launch (per partition
+// one kernel whose wave w owns chunk w of the batch, plus a one-workgroup
+// fix-up launch for kernels that carry state across chunks; optionally
+// captured into a hipGraph) and the batched plan interface.
 #include "fused.h"
 
 #include "bsdata_io.h"
@@ -111,9 +112,18 @@ std::string default_arch() {
 
 // hipRTC options of every generated unit (after --offload-arch).  Packed
 // FP32 instructions stay disabled, as in the rest of the library.
+// BSZ_RTC_OPTIONS appends space-separated options (for experiments with the
+// compiler; they enter the cache key like the fixed ones).
 const std::vector<std::string>& rtc_options() {
-    static const std::vector<std::string> o = {"-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-                                               "-packed-fp32-ops"};
+    static const std::vector<std::string> o = [] {
+        std::vector<std::string> v = {"-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                                      "-packed-fp32-ops"};
+        if (const char* e = std::getenv("BSZ_RTC_OPTIONS")) {
+            std::istringstream in(e);
+            for (std::string t; in >> t;) v.push_back(t);
+        }
+        return v;
+    }();
     return o;
 }
 
