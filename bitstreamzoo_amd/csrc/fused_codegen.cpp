@@ -136,6 +136,16 @@ __device__ __forceinline__ void lane_st16(GAS u64* rowbase, u32 off16, u64x2 v) 
 __device__ __forceinline__ u64 ldr(const GAS u64* rowbase, long long off, long long w, u64 n) {
   return (w >= 0 && (u64)w < n) ? rowbase[off] : 0ull;
 }
+// An empty asm makes a value opaque to the optimizer.  The AMDGPU
+// SelectionDAG backend of this compiler (ROCm 7.2 LLVM; gfx90a, gfx942 and
+// gfx950, at -O1 and -O3) miscompiles a 64-bit OR whose operand has a 32-bit
+// half known to be all ones and more than one use: the OR takes that half as
+// zero.  `~(x >> 61)` used twice, `x | 0xffffffff00000000` used twice, both
+// give a wrong half.  Such halves arise here only from values the compiler
+// can partly evaluate: an advance or a sum of a value it has proved constant
+// (say `b ^ b`).  Every advance and sum is therefore passed through opq(), so
+// that no stream value carries a known half.
+__device__ __forceinline__ u64 opq(u64 x) { __asm__("" : "+v"(x)); return x; }
 __device__ __forceinline__ u64 fsl(u64 cur, u64 lower, u32 b) { return b ? ((cur << b) | (lower >> (64 - b))) : cur; }
 __device__ __forceinline__ u64 fsr(u64 cur, u64 upper, u32 b) { return b ? ((cur >> b) | (upper << (64 - b))) : cur; }
 __device__ __forceinline__ u32 wave_sum(u32 v) {
@@ -580,7 +590,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                 std::string e = x.op == NOp::Shl
                     ? "fsl(" + word(-q) + ", " + (b ? word(-q - 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)"
                     : "fsr(" + word(q) + ", " + (b ? word(q + 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)";
-                s << "      const u64 y" << n << "_" << u << " = ok" << u << " ? " << e << " : 0ull;\n";
+                s << "      const u64 y" << n << "_" << u << " = ok" << u << " ? opq(" << e << ") : 0ull;\n";
             }
         }
     };
@@ -642,8 +652,8 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                     s << "        const u64 P" << t << " = G" << t << " | ballot(s" << t << " == ~0ull);\n";
                     s << "        const u64 t" << t << " = G" << t << " + P" << t << ";\n";
                     s << "        const u64 u" << t << " = t" << t << " + " << st << ";\n";
-                    s << "        const u64 " << v << " = s" << t << " + (((u" << t << " ^ G" << t << " ^ P" << t
-                      << ") >> lane) & 1ull);\n";
+                    s << "        const u64 " << v << " = opq(s" << t << " + (((u" << t << " ^ G" << t << " ^ P" << t
+                      << ") >> lane) & 1ull));\n";
                     s << "        " << st << " = (u64)((t" << t << " < G" << t << ") | (u" << t << " < t" << t << "));\n";
                     if (warm && track) {
                         // lanes that do not propagate a carry (they generate
@@ -667,8 +677,8 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
                         s << "        const u64 " << v << " = " << ref(x.a) << ";\n";
                     } else {
                         const std::string st = "st" + std::to_string(state_of.at(n));
-                        s << "        const u64 " << v << " = fsl(" << ref(x.a) << ", prev_word(" << ref(x.a) << ", " << st
-                          << "), " << x.k << "u);\n";
+                        s << "        const u64 " << v << " = opq(fsl(" << ref(x.a) << ", prev_word(" << ref(x.a) << ", " << st
+                          << "), " << x.k << "u));\n";
                         s << "        " << st << " = lane63(" << ref(x.a) << ");\n";
                         if (warm && track) {
                             // lane 0 takes the state; lane i > 0 takes operand lanes i and i-1
