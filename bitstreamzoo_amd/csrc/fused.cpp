@@ -223,11 +223,13 @@ KernelRegs kernel_regs(const std::vector<char>& code) {
 
 // A generated kernel runs only if its code object shows that every value stays
 // in the architectural VGPRs: no AGPRs, no scratch, and all three counts
-// present in the metadata (a missing count is treated as a refusal).  Two wide
-// random-DAG partitions whose kernels overflowed into AGPRs (one also into
-// scratch) produced 64-bit values with correct low and wrong high halves on
-// gfx950; the cause inside the compiled code was not found, so such kernels
-// are refused rather than run.
+// present in the metadata (a missing count is treated as a refusal).  A kernel
+// that overflows moves values through AGPRs or scratch on every row and loses
+// the streaming rate the partition was sized for, so it is regenerated or
+// refused rather than run.  (Two wide random-DAG partitions whose kernels
+// overflowed also gave 64-bit values with correct low and wrong high halves;
+// that is the signature of the OR miscompile the generated code now guards
+// against with opq(), see fused_codegen.cpp.)
 bool regs_clean(const KernelRegs& r) { return r.vgprs > 0 && r.agprs == 0 && r.scratch == 0; }
 
 void refuse_unclean(const PlanSpec& p, const std::vector<KernelRegs>& regs) {

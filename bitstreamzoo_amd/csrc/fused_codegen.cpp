@@ -186,7 +186,7 @@ constexpr int kMaxStateWords = 32;
 constexpr int kTrackNodes = 128;
 
 // Registers a generated kernel may use: the 256 architectural VGPRs at most
-// (kernels that overflowed into AGPRs gave wrong results on gfx950), fewer at
+// (values kept in AGPRs or scratch cost extra moves on every row), fewer at
 // a higher requested occupancy.
 int kernel_registers(const PlanOptions& opt) {
     const int occ = opt.occupancy > 0 ? opt.occupancy : 1;
@@ -270,10 +270,10 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     const int load_budget = std::max(0, regs_for_loads - 48 - NPOP_ - 2 * live_);
     const int u_cap = std::max(1, std::min(opt.rows_per_step, load_budget / std::max(1, 2 * loads_per_row)));
     const int WR = has_state ? std::max(1, opt.warmup_rows) : 0;
-    // Rows per step divide the chunk length.  (Letting a step run past the
-    // chunk, e.g. 3 rows per step over 16-row chunks, gave wrong results on
-    // MI355X for a kernel with 97 state words, with no difference in the
-    // generated source's semantics found; that form is not generated.)
+    // Rows per step divide the chunk length, so a step never runs past its
+    // chunk and needs no per-row bound inside the step.  (One kernel with 97
+    // state words gave wrong results in the form that ran past the chunk,
+    // with the same signature as the OR miscompile opq() guards against.)
     int U = 1;
     for (int d = 1; d <= std::min(u_cap, CR); ++d)
         if (CR % d == 0) U = d;

@@ -229,8 +229,8 @@ class Trial:
 
 def register_clean(info: dict) -> bool:
     """Every compiled kernel of the plan stays in its architectural VGPRs
-    (no AGPR use, no scratch); kernels that overflowed have given wrong
-    results, so the tuner does not use such schedules.  A kernel whose
+    (no AGPR use, no scratch); kernels that overflow spill values on every
+    row, so the tuner does not use such schedules.  A kernel whose
     counts are missing or unknown (-1) counts as unclean.  (The library
     itself refuses such plans; this is the tuner's own check of the info.)"""
     for k in info.get("kernels", []):

@@ -161,8 +161,8 @@ def test_string_results_come_from_one_complete_call():
 
 def test_overflowing_kernel_is_regenerated_smaller():
     """Two rows of the adder's 65 inputs in flight at occupancy 1 compile to a
-    kernel that uses AGPRs (kernels like it gave wrong high halves of 64-bit
-    values on MI355X).  The library never runs such a kernel: it regenerates
+    kernel that uses AGPRs, which spills values on every row.  The library
+    never runs such a kernel: it regenerates
     it with fewer rows per step, here one, which fits the 256 VGPRs."""
     w = workloads.adder32(1 << 16, seed=11)
     prog, case = api.Program(w.source), api.Case.from_bscase(w.cases[0])
