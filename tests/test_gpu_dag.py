@@ -64,19 +64,39 @@ def test_dag_batch_of_uneven_texts():
 ROWS_SWEEP = [(16, 2), (16, 3), (16, 4), (16, 5), (16, 6), (16, 7), (4, 3), (8, 3), (12, 3)]
 
 
-@pytest.mark.parametrize("chunk_rows,rows_per_step", ROWS_SWEEP)
-@pytest.mark.parametrize("occ", [1, 0])
-def test_rows_per_step_sweep_wide_kernel(chunk_rows, rows_per_step, occ):
-    """A wide partition (max_live 160: ~100 stateful ops per kernel) with
-    every rows-per-step count, including ones that do not divide the chunk."""
+def sweep_options(chunk_rows, rows_per_step, occ):
+    return {"chunk_rows": chunk_rows, "rows_per_step": rows_per_step, "occupancy": occ, "max_live": 160,
+            "max_ops": 4000}
+
+
+@pytest.fixture(scope="module")
+def sweep_program():
+    """The wide-partition program of the sweep, its oracle outputs, and every
+    schedule of the sweep compiled up front by 16 worker processes (a
+    schedule the library refuses is left to the test, which then skips the
+    run)."""
     from bitstreamzoo_amd import schedsearch
     spec = schedsearch.random_specs(5, 101, nodes=(2000, 5000), log2_bytes=(23, 25))[3]
     p = dag_gen.generate(spec.nodes, seed=spec.seed)
     case = dag_gen.text_case(64 * 64 * 40 + 123, seed=spec.seed + 7)
-    exp = oracle_outputs(p, case)
+    for occ in (1, 0):
+        for cr, rps in ROWS_SWEEP:
+            try:
+                tuning.precompile(p.source, [case], [sweep_options(cr, rps, occ)], processes=16)
+            except Exception as e:  # refused schedules are checked by plan_or_refusal below
+                if "schedule refused" not in str(e):
+                    raise
+    return p, case, oracle_outputs(p, case)
+
+
+@pytest.mark.parametrize("chunk_rows,rows_per_step", ROWS_SWEEP)
+@pytest.mark.parametrize("occ", [1, 0])
+def test_rows_per_step_sweep_wide_kernel(sweep_program, chunk_rows, rows_per_step, occ):
+    """A wide partition (max_live 160: ~100 stateful ops per kernel) with
+    every rows-per-step count, including ones that do not divide the chunk."""
+    p, case, exp = sweep_program
     c = ctx()
-    opts = {"chunk_rows": chunk_rows, "rows_per_step": rows_per_step, "occupancy": occ, "max_live": 160,
-            "max_ops": 4000}
+    opts = sweep_options(chunk_rows, rows_per_step, occ)
     plan = plan_or_refusal(c, api.Program(p.source), tuning.shape_case([case]), opts)
     if plan is None:
         return  # refused: its kernels would overflow the VGPRs (never run)
@@ -93,7 +113,8 @@ def test_rows_per_step_sweep_wide_kernel(chunk_rows, rows_per_step, occ):
 
 WIDE_REGRESSIONS = [
     # (spec list args, index, schedule): wide partitions that once gave wrong
-    # outputs when their kernels exhausted the register file and spilled
+    # outputs (the high halves of shared 64-bit ORs, the miscompile that the
+    # generated code guards against) while their kernels also spilled
     (dict(n=32, seed=201, nodes=(500, 5000), log2_bytes=(16, 22)), 25,
      {"chunk_rows": 4, "max_live": 160, "max_ops": 1000, "occupancy": 1, "rows_per_step": 1}),
     (dict(n=5, seed=101, nodes=(2000, 5000), log2_bytes=(23, 25)), 3,
