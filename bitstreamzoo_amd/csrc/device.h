@@ -154,11 +154,6 @@ void upload_stream(hipStream_t s, uint64_t* dst, const uint64_t* host, uint64_t 
 void stage_h2d(hipStream_t s, void* dst, const void* src, size_t bytes);
 // Device-to-host copy through the same buffers; synchronous.
 void stage_d2h(hipStream_t s, void* dst, const void* src, size_t bytes);
-// Before a fused plan launch: for each of n_ctl control blocks of 4 words
-// (ticket, done, epoch, mismatches) clear ticket/done/mismatches and advance
-// the epoch; clear n_pop popcount accumulators.
-// Advances a fused plan's run counter (ctl[0]).
-void launch_plan_epoch(hipStream_t s, unsigned* ctl);
 // Copy-engine bandwidth probe kernel: dst[i] = src[i] over n words.
 void launch_copy_words(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t n);
 

@@ -48,6 +48,7 @@ FusedOptions FusedOptions::from_env() {
     o.store_order = env_int("BSZ_STORE_ORDER", o.store_order);
     o.nontemporal = env_int("BSZ_NONTEMPORAL", o.nontemporal);
     o.interleave = env_int("BSZ_INTERLEAVE", o.interleave);
+    o.persistent = env_int("BSZ_PERSISTENT", o.persistent);
     o.max_ops = env_int("BSZ_MAX_OPS", o.max_ops);
     o.max_live = env_int("BSZ_MAX_LIVE", o.max_live);
     o.occupancy = env_int("BSZ_OCCUPANCY", o.occupancy);
@@ -77,10 +78,12 @@ FusedOptions FusedOptions::from_json_obj(const Json& j) {
     get("max_ops", o.max_ops);
     get("max_live", o.max_live);
     get("occupancy", o.occupancy);
+    get("persistent", o.persistent);
     get("compile", o.compile);
     get("compile_shard", o.compile_shard);
     get("compile_shards", o.compile_shards);
     get("device_if", o.device_if);
+    if (const Json* v = j.get("spin_limit")) o.spin_limit = v->as_int(o.spin_limit);
     if (const Json* v = j.get("cache_dir")) o.cache_dir = v->s;
     return o;
 }
@@ -98,6 +101,7 @@ PlanOptions to_plan_options(const FusedOptions& f) {
     p.max_ops = std::max(1, f.max_ops);
     p.max_live = std::max(4, f.max_live);
     p.occupancy = f.occupancy <= 0 ? 0 : std::min(8, f.occupancy);
+    p.persistent = f.persistent != 0;
     return p;
 }
 
@@ -510,6 +514,8 @@ public:
     ModuleP mod;
     uint32_t n_inst = 0, n_chunks = 0;
     uint64_t chunk_rows = 16;
+    uint32_t spin_limit = kSpinLimit;   // polls of a look-back wait before it gives up
+    unsigned* err_ext = nullptr;        // error word kept outside the arena (an engine's, across flushes)
 
     ~PlanRT() { release(); }
 
@@ -551,14 +557,10 @@ public:
         size_t total_pop = 0;
         off_inst_ = a.take(sizeof(InstDesc) * n_inst);
         off_chunk_inst_ = n_inst > 1 ? a.take(4ull * n_chunks) : 0;
-        off_ctl_ = a.take(4ull * (4 + 4 * K));
-        // unproven-chunk bitmaps of the stateful kernels, one region (cleared
-        // with the ctl block whenever the layout is set)
-        unp_words_ = (n_chunks + 31) / 32;
-        size_t n_stateful = 0;
-        for (size_t k = 0; k < K; ++k) n_stateful += P.kernels[k].n_state ? 1 : 0;
-        off_unp_ = a.take(4ull * std::max<size_t>(1, unp_words_ * n_stateful));
-        unp_bytes_ = 4ull * std::max<size_t>(1, unp_words_ * n_stateful);
+        off_ctl_ = a.take(ctl_bytes());
+        // run numbers of the chunks, cleared with the ctl block whenever the
+        // layout is set
+        off_runs_ = a.take(4ull * std::max<uint32_t>(1, n_chunks));
         for (size_t k = 0; k < K; ++k) total_pop += P.kernels[k].pops.size() * n_inst;
         uint32_t max_chunks = 1;
         for (const InstDesc& d : inst_h_) max_chunks = std::max(max_chunks, d.n_chunks);
@@ -566,7 +568,7 @@ public:
         while (spread_log_ < (uint32_t)kMaxPopSpreadLog && (4u << spread_log_) < max_chunks) ++spread_log_;
         pop_stride_ = total_pop << spread_log_;
         off_pop_ = a.take(8ull * std::max<size_t>(1, 2 * pop_stride_));
-        size_t pop_cursor = 0, unp_cursor = 0;
+        size_t pop_cursor = 0;
         for (size_t k = 0; k < K; ++k) {
             const KernelSpec& ks = P.kernels[k];
             KRes& r = kres_[k];
@@ -578,12 +580,9 @@ public:
             r.out_tab = a.take(8ull * std::max<size_t>(1, r.nout * n_inst));
             r.pop = off_pop_ + (8ull << spread_log_) * pop_cursor;
             pop_cursor += (size_t)r.npop * n_inst;
-            if (r.nst) {
-                r.unproven = off_unp_ + 4ull * unp_words_ * unp_cursor++;
-                r.st_u = a.take(8ull * n_chunks * r.nst);
-                r.st_t = a.take(8ull * n_chunks * r.nst);
-                r.partial = a.take(4ull * n_chunks * std::max(1, r.npop));
-            }
+            // tagged end-state halves: a stale tag never matches a run
+            // (tags are run + 1, and the region starts at zero)
+            if (r.nst) r.st_t = a.take(16ull * n_chunks * r.nst);
         }
         total_pop_ = total_pop;
         if (a.bytes > dev_bytes_) {
@@ -593,10 +592,13 @@ public:
             BSZ_HIP(hipMemsetAsync(dev_, 0, a.bytes, s));
             dev_bytes_ = a.bytes;
         } else {
-            // epoch 0 and both popcount banks clear for the new layout
-            BSZ_HIP(hipMemsetAsync(ptr(off_ctl_), 0, 4ull * (4 + 4 * K), s));
-            BSZ_HIP(hipMemsetAsync(ptr(off_unp_), 0, unp_bytes_, s));
+            // no run yet, no error, both popcount banks clear, and no stale
+            // tag that a run could match, for the new layout
+            BSZ_HIP(hipMemsetAsync(ptr(off_ctl_), 0, ctl_bytes(), s));
+            BSZ_HIP(hipMemsetAsync(ptr(off_runs_), 0, 4ull * std::max<uint32_t>(1, n_chunks), s));
             BSZ_HIP(hipMemsetAsync(ptr(off_pop_), 0, 8ull * std::max<size_t>(1, 2 * pop_stride_), s));
+            for (const KRes& r : kres_)
+                if (r.nst) BSZ_HIP(hipMemsetAsync(ptr(r.st_t), 0, 16ull * n_chunks * r.nst, s));
         }
         in_tab_h_.assign(K, {});
         out_tab_h_.assign(K, {});
@@ -636,13 +638,12 @@ public:
         tables_dirty_ = false;
     }
 
-    // Every kernel, asynchronously on s: the main launch, then, for kernels
-    // with state, the fix-up launch.  The run's last launch advances the epoch
-    // (a separate one-thread launch when the last kernel has no state).
+    // Every kernel, asynchronously on s, one launch each; the last advances
+    // the run numbers of the chunks.
     void launch(hipStream_t s) {
         const PlanSpec& P = *spec;
         const int wpb = P.opt.waves_per_block;
-        const uint32_t blocks = (n_chunks + wpb - 1) / wpb;
+        const uint32_t all_blocks = (n_chunks + wpb - 1) / wpb;
         const size_t K = P.kernels.size();
         const int64_t before = launches_;
         for (size_t k = 0; k < K; ++k) {
@@ -653,43 +654,44 @@ public:
             A.pop_acc = (unsigned long long*)ptr(r.pop);
             A.inst = (const InstDesc*)ptr(off_inst_);
             A.chunk_inst = n_inst > 1 ? (const uint32_t*)ptr(off_chunk_inst_) : nullptr;
-            A.st_u = r.nst ? (unsigned long long*)ptr(r.st_u) : nullptr;
             A.st_t = r.nst ? (unsigned long long*)ptr(r.st_t) : nullptr;
-            A.partial = r.nst ? (unsigned*)ptr(r.partial) : nullptr;
-            A.ctl = (unsigned*)ptr(off_ctl_);
-            A.kstat = (unsigned*)ptr(off_ctl_ + 4ull * (4 + 4 * k));
-            A.unproven = r.nst ? (unsigned*)ptr(r.unproven) : nullptr;
+            A.runs = (unsigned*)ptr(off_runs_);
+            A.kstat = (unsigned*)ptr(off_ctl_ + 4ull * (4 + 8 * k));
+            A.err = err_word();
             A.pop_stride = pop_stride_;
             A.n_chunks = n_chunks;
             A.n_inst = n_inst;
-            A.mode = 0;
-            A.last = 0;
+            A.last = k + 1 == K ? 1u : 0u;
             A.spread_log = spread_log_;
+            A.spin_limit = spin_limit;
+            const uint32_t blocks = P.opt.persistent ? std::min(all_blocks, resident_blocks(k, wpb)) : all_blocks;
+            A.grid_waves = blocks * (uint32_t)wpb;
             void* params[] = {&A};
             BSZ_HIP(hipModuleLaunchKernel(mod->fns[k], blocks, 1, 1, 64 * wpb, 1, 1, 0, s, params, nullptr));
-            ++launches_;
-            if (r.nst) {
-                A.mode = 1;
-                A.last = k + 1 == K ? 1u : 0u;
-                BSZ_HIP(hipModuleLaunchKernel(mod->fns[k], 1, 1, 1, kFixThreads, 1, 1, 4u * (kFixBitmapWords + 1), s,
-                                              params, nullptr));
-                ++launches_;
-            }
-        }
-        if (K == 0 || !kres_[K - 1].nst) {
-            launch_plan_epoch(s, (unsigned*)ptr(off_ctl_));
             ++launches_;
         }
         per_run_launches_ = launches_ - before;
     }
 
-    // Launches of one run: each kernel, a fix-up per stateful kernel, and the
-    // epoch kernel when the last kernel has no fix-up.
-    int64_t launches_per_run() const {
-        const size_t K = spec->kernels.size();
-        int64_t n = (int64_t)K;
-        for (const KRes& r : kres_) n += r.nst ? 1 : 0;
-        if (K == 0 || !kres_[K - 1].nst) ++n;
+    int64_t launches_per_run() const { return (int64_t)spec->kernels.size(); }
+
+    // Workgroups of kernel k the GPU keeps resident at once: whole
+    // workgroups of `wpb` waves per CU, at the waves per SIMD its compiled
+    // VGPR count allows (8-register granules, at most 8 waves).
+    uint32_t resident_blocks(size_t k, int wpb) const {
+        const int vg = k < mod->regs.size() && mod->regs[k].vgprs > 0 ? mod->regs[k].vgprs : 256;
+        const int per_simd = std::max(1, std::min(8, 512 / ((vg + 7) & ~7)));
+        const uint32_t per_cu = (uint32_t)std::max(1, 4 * per_simd / wpb);
+        return (uint32_t)device_cus() * per_cu;
+    }
+    static int device_cus() {
+        static const int n = [] {
+            int dev = 0, v = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+                return 256;
+            return v;
+        }();
         return n;
     }
 
@@ -708,11 +710,9 @@ public:
     void read_pops(hipStream_t s, std::vector<unsigned long long>& out) {
         out.assign(total_pop_, 0);
         if (!total_pop_) return;
-        uint32_t epoch = 0;
-        BSZ_HIP(hipMemcpyAsync(&epoch, ptr(off_ctl_), 4, hipMemcpyDeviceToHost, s));
-        BSZ_HIP(hipStreamSynchronize(s));
-        if (epoch == 0) return;  // no run yet
-        const size_t bank = (epoch - 1) & 1u;
+        const uint32_t runs = completed_runs(s);
+        if (runs == 0) return;  // no run yet
+        const size_t bank = (runs - 1) & 1u;
         std::vector<unsigned long long> slots(pop_stride_);
         BSZ_HIP(hipMemcpyAsync(slots.data(), ptr(off_pop_ + 8ull * bank * pop_stride_), 8ull * pop_stride_,
                                hipMemcpyDeviceToHost, s));
@@ -732,13 +732,28 @@ public:
         }
         throw RunError("popcount not in plan");
     }
-    // stat `which` of kernel k after the last run (sync): 0 chunks whose
-    // speculative start state was wrong, 3 chunks whose warm-up did not prove
-    // their start state exact
+    // Runs completed over chunk 0 (every chunk's count is the same between
+    // runs), after checking the plan's error word (sync).
+    uint32_t completed_runs(hipStream_t s) {
+        uint32_t v[2] = {0, 0};
+        BSZ_HIP(hipMemcpyAsync(&v[0], err_word(), 4, hipMemcpyDeviceToHost, s));
+        BSZ_HIP(hipMemcpyAsync(&v[1], ptr(off_runs_), 4, hipMemcpyDeviceToHost, s));
+        BSZ_HIP(hipStreamSynchronize(s));
+        if (v[0] & 1u)
+            throw RunError("a fused kernel's look-back wait for its predecessor chunk reached its bound; "
+                           "the outputs of that run are not exact");
+        return v[1];
+    }
+    // counter `which` of kernel k for the last run (sync): 0 chunks run again
+    // from their predecessor's end state, 1 chunks whose warm-up did not
+    // prove their start state, 2 look-back waits that reached the bound
     uint32_t kernel_stat(hipStream_t s, size_t k, int which) {
         if (k >= kres_.size()) throw RunError("no kernel " + std::to_string(k) + " in this plan");
+        const uint32_t runs = completed_runs(s);
+        if (runs == 0) return 0;
         uint32_t v = 0;
-        BSZ_HIP(hipMemcpyAsync(&v, ptr(off_ctl_ + 4ull * (4 + 4 * k) + 4ull * which), 4, hipMemcpyDeviceToHost, s));
+        const size_t at = 4 + 8 * k + 4 * ((runs - 1) & 1u) + (size_t)which;
+        BSZ_HIP(hipMemcpyAsync(&v, ptr(off_ctl_ + 4ull * at), 4, hipMemcpyDeviceToHost, s));
         BSZ_HIP(hipStreamSynchronize(s));
         return v;
     }
@@ -750,12 +765,12 @@ private:
     struct KRes {
         size_t nin = 0, nout = 0;
         int npop = 0, nst = 0;
-        size_t in_tab = 0, out_tab = 0, pop = 0, st_u = 0, st_t = 0, partial = 0, unproven = 0;
+        size_t in_tab = 0, out_tab = 0, pop = 0, st_t = 0;
     };
     void* dev_ = nullptr;
     size_t dev_bytes_ = 0;
-    size_t off_inst_ = 0, off_chunk_inst_ = 0, off_ctl_ = 0, off_pop_ = 0, total_pop_ = 0, pop_stride_ = 0;
-    size_t off_unp_ = 0, unp_words_ = 0, unp_bytes_ = 0;
+    size_t off_inst_ = 0, off_chunk_inst_ = 0, off_ctl_ = 0, off_runs_ = 0, off_pop_ = 0, total_pop_ = 0,
+           pop_stride_ = 0;
     uint32_t spread_log_ = 0;
     std::vector<KRes> kres_;
     std::vector<InstDesc> inst_h_;
@@ -767,6 +782,9 @@ private:
     int64_t launches_ = 0, per_run_launches_ = 0;
 
     char* ptr(size_t off) const { return static_cast<char*>(dev_) + off; }
+    // error word, 3 spare, then 8 counters per kernel
+    size_t ctl_bytes() const { return 4ull * (4 + 8 * spec->kernels.size()); }
+    unsigned* err_word() const { return err_ext ? err_ext : (unsigned*)ptr(off_ctl_); }
 };
 
 // Binds the buffers of one instance: sources, segment outputs and slots.
@@ -813,6 +831,8 @@ public:
 
     ~FusedEngine() override {
         if (!recording_) (void)hipStreamSynchronize(s_);
+        if (slot_) (void)hipFree(slot_);
+        if (err_) (void)hipFree(err_);
     }
 
     // ── Engine ──
@@ -925,6 +945,19 @@ public:
             return;
         }
         stage_d2h(s_, host, a->buf->ptr, g_.lw * 8);
+        check_waits();
+    }
+
+    // A look-back wait of any kernel since the engine began that gave up
+    // leaves outputs that are not exact: report it (sync).
+    void check_waits() {
+        if (!err_) return;
+        unsigned v = 0;
+        BSZ_HIP(hipMemcpyAsync(&v, err_, 4, hipMemcpyDeviceToHost, s_));
+        BSZ_HIP(hipStreamSynchronize(s_));
+        if (v & 1u)
+            throw RunError("a fused kernel's look-back wait for its predecessor chunk reached its bound; "
+                           "the outputs of that run are not exact");
     }
 
     void flush() override;
@@ -974,6 +1007,7 @@ private:
     DevPool pool_;
     SV zero_, ones_;
     unsigned long long* slot_ = nullptr;
+    unsigned* err_ = nullptr;   // sticky error word of every plan this engine runs
 
     uint64_t seg_ = 1;
     std::vector<GNode> nodes_;
@@ -1083,6 +1117,11 @@ void FusedEngine::flush() {
     if (!g.outputs.empty() || !g.pops.empty()) {
         CachedPlan& cp = plan_for(g);
         PlanRT& rt = *cp.rt;
+        if (!err_) {
+            BSZ_HIP(hipMalloc(&err_, 4));
+            BSZ_HIP(hipMemsetAsync(err_, 0, 4, s_));
+        }
+        rt.err_ext = err_;
         rt.configure({g_}, s_);
         std::vector<const uint64_t*> src(sources_.size());
         for (size_t i = 0; i < sources_.size(); ++i) src[i] = sources_[i]->ptr;
@@ -1103,7 +1142,7 @@ void FusedEngine::flush() {
             bind_instance(rt, 0, src, seg_out, slots);
             rt.upload_tables(s_);
             rt.launch(s_);
-            launches_ += 1 + (int64_t)cp.spec->kernels.size();
+            launches_ += (int64_t)cp.spec->kernels.size();
             kernels_run_ += (int64_t)cp.spec->kernels.size();
         }
         if (!pops_.empty()) {
@@ -1298,6 +1337,7 @@ void FusedPlan::load() {
         I.rt = std::make_unique<PlanRT>();
         I.rt->spec = I.spec;
         I.rt->mod = I.mod;
+        if (I.fopt.spin_limit >= 0) I.rt->spin_limit = (uint32_t)std::min<int64_t>(I.fopt.spin_limit, kSpinLimit);
     }
 }
 
@@ -1480,7 +1520,7 @@ int64_t FusedPlan::read_int(int inst, const std::string& name, bool refresh) {
 }
 
 uint32_t FusedPlan::mismatches(int kernel) { return impl_->rt->mismatches(impl_->s, kernel); }
-uint32_t FusedPlan::unproven(int kernel) { return impl_->rt->kernel_stat(impl_->s, kernel, 3); }
+uint32_t FusedPlan::unproven(int kernel) { return impl_->rt->kernel_stat(impl_->s, kernel, 1); }
 
 std::string FusedPlan::describe_json(const std::string& options_array) const {
     const Json arr = json_parse(options_array);
@@ -1570,7 +1610,7 @@ std::string FusedPlan::info_json() const {
       << ",\"chunk_rows\":" << I.spec->opt.chunk_rows << ",\"warmup_rows\":" << I.spec->opt.warmup_rows
       << ",\"waves_per_block\":" << I.spec->opt.waves_per_block << ",\"rows_per_step\":" << I.spec->opt.rows_per_step
       << ",\"max_ops\":" << I.spec->opt.max_ops << ",\"max_live\":" << I.spec->opt.max_live
-      << ",\"occupancy\":" << I.spec->opt.occupancy
+      << ",\"occupancy\":" << I.spec->opt.occupancy << ",\"persistent\":" << (I.spec->opt.persistent ? 1 : 0)
       << ",\"n_chunks\":" << (I.rt ? I.rt->n_chunks : 0) << ",\"device_bytes\":" << (I.rt ? I.rt->device_bytes() : 0)
       << "}";
     return o.str();

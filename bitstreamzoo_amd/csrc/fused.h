@@ -22,12 +22,15 @@ struct FusedOptions {
     int max_ops = 4000;         // fusion depth: DAG nodes per kernel
     int max_live = 96;          // live 64-bit values per lane per kernel
     int occupancy = 0;          // waves per SIMD the register budget allows (512/occupancy VGPRs); 0 = auto
+    int persistent = 0;         // 1: each kernel's grid holds only resident waves, each striding over the chunks
     std::string cache_dir;      // code-object cache ("" = $BSZ_CACHE_DIR or ~/.cache/bsz_hip)
     int compile = 1;            // compile-only: 0 partitions and generates without running hipRTC
     int compile_shard = 0;      // compile-only: compile the kernels k with k % compile_shards == compile_shard
     int compile_shards = 1;     //   (several processes fill the cache of one plan in parallel)
     int device_if = 0;          // compile-only: record `if` bodies as device selections (inspection of the plan
                                 //   bsz_run would build; a recorded plan with selections is never run)
+    int64_t spin_limit = -1;    // polls of a look-back wait before it gives up (-1: the library's bound);
+                                //   a run whose wait gave up reports an error instead of outputs
     bool verbose = false;
     static FusedOptions from_env();
     // Env defaults overridden by the keys of a flat JSON object

@@ -43,8 +43,8 @@ struct InstDesc { u64 rows; u64 lw; u64 lastmask; u64 opw; u64 opmask; u32 chunk
 struct KArgs {
   const u64* const* in; u64* const* out; u64* pop_acc;
   const InstDesc* inst; const u32* chunk_inst;
-  u64* st_u; u64* st_t; u32* partial; u32* ctl; u32* kstat; u32* unproven;
-  u64 pop_stride; u32 n_chunks; u32 n_inst; u32 mode; u32 last; u32 spread_log; u32 pad_;
+  u64* st_t; u32* runs; u32* kstat; u32* err;
+  u64 pop_stride; u32 n_chunks; u32 n_inst; u32 last; u32 spread_log; u32 spin_limit; u32 grid_waves;
 };
 #define AGENT __HIP_MEMORY_SCOPE_AGENT
 // Binding tables are never written while a kernel runs: reading them through
@@ -193,6 +193,44 @@ int kernel_registers(const PlanOptions& opt) {
     return std::min(256, 512 / occ);
 }
 
+// Every kernel reads the run number of its chunk (how many runs of the plan
+// have completed over it): its parity selects this run's popcount bank, and
+// a stateful kernel tags its published end states with run + 1.  The wave
+// of the chunk clears its share of the other bank for the next run, and the
+// wave of chunk 0 clears the next run's counters of a stateful kernel.
+void emit_run_prologue(std::ostringstream& s, int NPOP, bool has_state) {
+    s << "  const u32 run = u32_at(A.runs, c);\n";
+    s << "  const u32 ep = run & 1u;\n";
+    if (NPOP) s << "  u64* const bank = A.pop_acc + (ep ? A.pop_stride : 0ull);\n";
+    if (has_state) {
+        s << "  const u32 tag = run + 1u;\n";
+        s << "  u32* const kst = A.kstat + (ep ? 4u : 0u);\n";
+        s << "  if (c == 0u && lane < 4u) A.kstat[(ep ? 0u : 4u) + lane] = 0u;\n";
+    }
+}
+
+// Which chunks a wave runs.  By default wave w of the grid owns chunk w.
+// A persistent kernel's grid holds no more waves than the GPU keeps resident
+// at the kernel's occupancy, and wave w runs chunks w, w + G, w + 2G, ...
+// in increasing order: every wave is resident, so the lowest unfinished
+// chunk, whose predecessor is finished, always makes progress, whatever the
+// order in which the hardware dispatches workgroups.
+void emit_chunk_loop(std::ostringstream& s, const PlanOptions& opt, int WPB) {
+    if (opt.persistent) {
+        s << "  for (u32 c = blockIdx.x * " << WPB << "u + uni(threadIdx.x >> 6); c < A.n_chunks; c += A.grid_waves) {\n";
+    } else {
+        s << "  {\n";
+        s << "  const u32 c = blockIdx.x * " << WPB << "u + uni(threadIdx.x >> 6);\n";
+        s << "  if (c >= A.n_chunks) return;\n";
+    }
+}
+
+// The plan's last kernel ends the run over each chunk by advancing the
+// chunk's run number, after everything else the wave wrote.
+void emit_run_epilogue(std::ostringstream& s) {
+    s << "  if (A.last && lane == 0u) A.runs[c] = run + 1u;\n";
+}
+
 // Generates one kernel; fills ks.source and ks.n_state.
 void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool emit) {
     std::set<int32_t> in_kernel(ks.nodes.begin(), ks.nodes.end());
@@ -209,7 +247,8 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     ks.n_state = n_state;
     const bool has_state = n_state > 0;
     // Lane-exactness tracking in warm-up rows costs a scalar per node; large
-    // kernels skip it (their chunks are all checked by the fix-up launch).
+    // kernels skip it (their chunks after the first warm-up rows all look
+    // back at their predecessor's end state).
     const bool track = has_state && (int)ks.nodes.size() <= kTrackNodes;
 
     // which graph nodes need a plain value of an input (loaded per row)
@@ -311,7 +350,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     }
     ks.occupancy = occ;
     const int WPB = opt.waves_per_block;
-    const int LB = has_state ? std::max(64 * WPB, kFixThreads) : 64 * WPB;
+    const int LB = 64 * WPB;
     ks.launch_bounds = LB;
     if (!emit) return;  // shape only (register budget, rows per step), no source
     if (wide) {
@@ -320,10 +359,8 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "  const u32 lane = lane_id();\n";
         s << "  const u32 off16 = lane * 16u;\n";
         s << "  const u64x2 zero2 = {0ull, 0ull};\n";
-        if (NPOP || NPRED) s << "  const u32 ep = u32_at(A.ctl, 0u) & 1u;\n";
-        if (NPOP) s << "  u64* const bank = A.pop_acc + (ep ? A.pop_stride : 0ull);\n";
-        s << "  const u32 c = blockIdx.x * " << WPB << "u + uni(threadIdx.x >> 6);\n";
-        s << "  if (c >= A.n_chunks) return;\n";
+        emit_chunk_loop(s, opt, WPB);
+        emit_run_prologue(s, NPOP, false);
         if (NPOP) {
             s << "  {\n    u64* const other = A.pop_acc + (ep ? 0ull : A.pop_stride);\n";
             s << "    const u64 n_slots = ((u64)A.n_inst * " << NPOP << "ull) << A.spread_log;\n";
@@ -416,131 +453,37 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
             s << "    if (lane == 0) __hip_atomic_fetch_add(&bank[(((u64)ii * " << NPOP << "u + " << p
               << "u) << A.spread_log) + (c & ((1u << A.spread_log) - 1u))], (u64)tot, __ATOMIC_RELAXED, AGENT);\n  }\n";
         }
+        emit_run_epilogue(s);
+        s << "  }\n";  // chunk
         s << "}\n";
         ks.source = s.str();
         return;
     }
     const std::string NS = std::to_string(n_state) + "u";
+    const std::string NS2 = std::to_string(2 * n_state) + "u";
     s << "extern \"C\" __global__ __launch_bounds__(" << LB << ") __attribute__((amdgpu_waves_per_eu(" << occ
       << "))) void bsz_kernel(KArgs A) {\n";
     s << "  const u32 lane = lane_id();\n";
     s << "  const u32 off8 = lane * 8u;\n";
-    s << "  const bool fix = A.mode != 0u;\n";
-    // popcount slots of this run, and of the next one (cleared here)
-    if (NPOP || NPRED) s << "  const u32 ep = u32_at(A.ctl, 0u) & 1u;\n";
-    if (NPOP) s << "  u64* const bank = A.pop_acc + (ep ? A.pop_stride : 0ull);\n";
-    s << "  u32 c = 0;\n";
-    if (has_state) {
-        // mode 1 (one workgroup).  Chunks whose warm-up proved their start
-        // state exact need nothing; when the main launch left no chunk
-        // unproven the launch ends at once.  Otherwise every unproven chunk
-        // whose speculative start state differs from its predecessor's end
-        // state is marked, in parallel, and one wave recomputes the marked
-        // chunks in order.
-        // dynamic LDS: only the fix-up launch asks for it, so the main launch's
-        // occupancy is not limited by the bitmap
-        s << "  extern __shared__ u32 s_dyn[];\n";
-        s << "  u32* const s_bits = s_dyn;\n  u32& s_any = s_dyn[" << kFixBitmapWords << "];\n";
-        s << "  u32 p_next = 1u, n_re = 0u, n_mis = 0u;\n  bool casc = false, use_bits = true;\n";
-        // end state of the last recomputed chunk, kept in registers for its
-        // successor (the walk is in increasing order, so no reload is needed)
-        s << "  u32 last_c = 0xfffffffeu;\n";
-        for (auto& kv : state_of) s << "  u64 st" << kv.second << " = 0ull;\n";
-        s << "  if (fix) {\n";
-        s << "    const u32 n_unproven = aload32(&A.kstat[2]);\n";
-        s << "    if (n_unproven == 0u) {\n";
-        s << "      if (threadIdx.x == 0) {\n";
-        s << "        astore32(&A.kstat[0], 0u);\n        astore32(&A.kstat[1], 0u);\n        astore32(&A.kstat[3], 0u);\n";
-        s << "        if (A.last) astore32(&A.ctl[0], aload32(&A.ctl[0]) + 1u);\n";
-        s << "      }\n";
-        s << "      return;\n";
-        s << "    }\n";
-        s << "    const bool fits = A.n_chunks <= 32u * " << kFixBitmapWords << "u;\n";
-        s << "    const u32 nbw = fits ? (A.n_chunks + 31u) / 32u : 0u;\n";
-        s << "    for (u32 i = threadIdx.x; i < nbw; i += blockDim.x) s_bits[i] = 0u;\n";
-        s << "    if (threadIdx.x == 0) s_any = 0u;\n";
-        s << "    __syncthreads();\n";
-        s << "    for (u32 x = 1u + threadIdx.x; fits && x < A.n_chunks; x += blockDim.x) {\n";
-        s << "      if (!((aload32(&A.unproven[x >> 5]) >> (x & 31u)) & 1u)) continue;\n";
-        s << "      const u32 xi = A.chunk_inst ? A.chunk_inst[x] : 0u;\n";
-        s << "      if (x == A.inst[xi].chunk_begin) continue;\n";
-        s << "      bool bad = false;\n";
-        s << "      for (u32 i = 0; i < " << NS << "; ++i)\n";
-        s << "        bad |= A.st_u[(u64)x * " << NS << " + i] != A.st_t[(u64)(x - 1u) * " << NS << " + i];\n";
-        s << "      if (bad) { atomicOr(&s_bits[x >> 5], 1u << (x & 31u)); atomicAdd(&s_any, 1u); }\n";
-        s << "    }\n";
-        s << "    __syncthreads();\n";
-        // the unproven marks and their count start from zero in the next run
-        s << "    for (u32 i = threadIdx.x; i < (A.n_chunks + 31u) / 32u; i += blockDim.x) astore32(&A.unproven[i], 0u);\n";
-        s << "    if (threadIdx.x == 0) { astore32(&A.kstat[2], 0u); astore32(&A.kstat[3], n_unproven); }\n";
-        s << "    if (threadIdx.x >= 64u) return;\n";
-        s << "    n_mis = s_any;\n";
-        s << "    use_bits = fits;\n";
-        s << "  } else {\n";
-    } else {
-        s << "  {\n";
-    }
-    // main launch: wave w of the grid owns chunk w
-    s << "    c = blockIdx.x * " << WPB << "u + uni(threadIdx.x >> 6);\n";
-    s << "    if (c >= A.n_chunks) return;\n";
+    emit_chunk_loop(s, opt, WPB);
+    emit_run_prologue(s, NPOP, has_state);
     if (NPOP) {
-        s << "    u64* const other = A.pop_acc + (ep ? 0ull : A.pop_stride);\n";
+        s << "  {\n    u64* const other = A.pop_acc + (ep ? 0ull : A.pop_stride);\n";
         s << "    const u64 n_slots = ((u64)A.n_inst * " << NPOP << "ull) << A.spread_log;\n";
-        s << "    for (u64 i = (u64)c * 64ull + lane; i < n_slots; i += (u64)A.n_chunks * 64ull) other[i] = 0ull;\n";
+        s << "    for (u64 i = (u64)c * 64ull + lane; i < n_slots; i += (u64)A.n_chunks * 64ull) other[i] = 0ull;\n  }\n";
     }
-    s << "  }\n";
-    s << "  for (;;) {\n";
-    if (has_state) {
-        // mode 1, first wave: the next chunk to recompute, in order: a marked
-        // chunk, or the successor of a recomputed chunk when its start state
-        // differs from the corrected end state
-        s << "    if (fix) {\n";
-        s << "      bool found = false;\n";
-        s << "      while (!found) {\n";
-        s << "        u32 cand = p_next;\n";
-        s << "        if (!casc && use_bits) {\n";
-        s << "          cand = 0xffffffffu;\n";
-        s << "          const u32 nbw = (A.n_chunks + 31u) >> 5;\n";
-        s << "          for (u32 wb = p_next >> 5; n_mis != 0u && wb < nbw; wb += 64u) {\n";
-        s << "            const u32 wi = wb + lane;\n";
-        s << "            u32 bits = wi < nbw ? s_bits[wi] : 0u;\n";
-        s << "            if (wi == (p_next >> 5)) bits &= ~0u << (p_next & 31u);\n";
-        s << "            const u64 hit = ballot(bits != 0u);\n";
-        s << "            if (hit) {\n";
-        s << "              const u32 l = (u32)__builtin_ctzll(hit);\n";
-        s << "              const u32 b = (u32)__builtin_amdgcn_readlane((int)bits, (int)l);\n";
-        s << "              cand = (wb + l) * 32u + (u32)__builtin_ctz(b);\n";
-        s << "              break;\n";
-        s << "            }\n";
-        s << "          }\n";
-        s << "        }\n";
-        s << "        if (cand >= A.n_chunks) goto fix_done;\n";
-        s << "        const u32 ci = A.chunk_inst ? u32_at(A.chunk_inst, cand) : 0u;\n";
-        s << "        bool need = false;\n";
-        s << "        if (cand != inst_at(A.inst, ci).chunk_begin) {\n";
-        s << "          const bool regs = cand == last_c + 1u;\n";
-        for (auto& kv : state_of)
-            s << "          need |= aload64(&A.st_u[(u64)cand * " << NS << " + " << kv.second << "u]) != (regs ? st"
-              << kv.second << " : aload64(&A.st_t[(u64)(cand - 1u) * " << NS << " + " << kv.second << "u]));\n";
-        s << "        }\n";
-        s << "        p_next = cand + 1u;\n";
-        s << "        casc = false;\n";
-        s << "        if (need) { c = cand; found = true; }\n";
-        s << "      }\n";
-        s << "    }\n";
-    }
-    s << "    const u32 ii = A.chunk_inst ? u32_at(A.chunk_inst, c) : 0u;\n";
-    s << "    const InstDesc D = inst_at(A.inst, ii);\n";
-    s << "    const u32 cc = c - D.chunk_begin;\n";
-    s << "    const u64 NW = D.rows * 64ull;\n";
-    for (int j = 0; j < NIN; ++j) s << "    const u64* __restrict__ in" << j << " = tab_in(A.in, (u64)ii * " << NINT << "u + " << j << "u);\n";
-    for (int j = 0; j < NOUT; ++j) s << "    u64* __restrict__ out" << j << " = tab_out(A.out, (u64)ii * " << NOUT << "u + " << j << "u);\n";
+    for (auto& kv : state_of) s << "  u64 st" << kv.second << " = 0ull;\n";
+    s << "  const u32 ii = A.chunk_inst ? u32_at(A.chunk_inst, c) : 0u;\n";
+    s << "  const InstDesc D = inst_at(A.inst, ii);\n";
+    s << "  const u32 cc = c - D.chunk_begin;\n";
+    s << "  const u64 NW = D.rows * 64ull;\n";
+    for (int j = 0; j < NIN; ++j) s << "  const u64* __restrict__ in" << j << " = tab_in(A.in, (u64)ii * " << NINT << "u + " << j << "u);\n";
+    for (int j = 0; j < NOUT; ++j) s << "  u64* __restrict__ out" << j << " = tab_out(A.out, (u64)ii * " << NOUT << "u + " << j << "u);\n";
     for (int j = 0; j < NPRED; ++j)
-        s << "    const bool pf" << j << " = pred_flag(tab_in(A.in, (u64)ii * " << NINT << "u + " << (NIN + j)
+        s << "  const bool pf" << j << " = pred_flag(tab_in(A.in, (u64)ii * " << NINT << "u + " << (NIN + j)
           << "u), A.pop_stride, ep, A.spread_log);\n";
-    s << "    const u64 r_begin = (u64)cc * " << CR << "ull;\n";
-    s << "    const u64 r_end = (r_begin + " << CR << "ull < D.rows) ? r_begin + " << CR << "ull : D.rows;\n";
-
+    s << "  const u64 r_begin = (u64)cc * " << CR << "ull;\n";
+    s << "  const u64 r_end = (r_begin + " << CR << "ull < D.rows) ? r_begin + " << CR << "ull : D.rows;\n";
     // Interleaved rows (kernels without state only): chunk cc of an instance
     // with n chunks takes rows cc, cc + n, cc + 2n, ..., so the waves running
     // at one time read one contiguous window of every stream rather than one
@@ -723,18 +666,26 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "      }\n";
     };
 
+    // The state words as this chunk publishes them: lane 2i holds the low and
+    // lane 2i+1 the high half of state word i, each under the run's tag in
+    // the upper 32 bits, so one 64-bit store per lane publishes a word half
+    // with the run it belongs to and a reader needs no fence.
+    auto emit_tagged = [&](const std::string& var) {
+        s << "    u32 " << var << "_v = 0u;\n";
+        for (auto& kv : state_of) {
+            const int i = kv.second;
+            s << "    " << var << "_v = lane == " << 2 * i << "u ? (u32)st" << i << " : " << var << "_v;\n";
+            s << "    " << var << "_v = lane == " << 2 * i + 1 << "u ? (u32)(st" << i << " >> 32) : " << var << "_v;\n";
+        }
+        s << "    const u64 " << var << " = ((u64)tag << 32) | " << var << "_v;\n";
+    };
     if (has_state) {
-        s << "    const u64 warm = (!fix && cc > 0) ? (r_begin < " << WR << "ull ? r_begin : " << WR << "ull) : 0ull;\n";
-        s << "    if (!fix) {\n";
-        for (auto& kv : state_of) s << "      st" << kv.second << " = 0ull;\n";
-        s << "    } else if (c != last_c + 1u) {\n";
-        for (auto& kv : state_of)
-            s << "      st" << kv.second << " = aload64(&A.st_t[(u64)(c - 1u) * " << NS << " + " << kv.second << "u]);\n";
-        s << "    }\n";
-        // Warm-up (main launch, chunks after an instance's first): the rows
-        // before the chunk, from a zero state, storing nothing.  A chunk
-        // whose warm-up proves every state exact is final; the others are
-        // marked for the fix-up launch.
+        // Warm-up (chunks after an instance's first): the rows before the
+        // chunk, from a zero state, storing nothing.  A warm-up that starts at
+        // the instance's first row reaches the true state; otherwise the
+        // tracking below may prove the state it reaches exact.
+        s << "    const u64 warm = cc > 0u ? (r_begin < " << WR << "ull ? r_begin : " << WR << "ull) : 0ull;\n";
+        s << "    u32 proven = 1u;\n";
         s << "    if (warm) {\n";
         if (track)
             for (auto& kv : state_of) s << "      u32 xs" << kv.second << " = 0u;\n";
@@ -742,29 +693,29 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         emit_loads(U, "r_begin");
         for (int u = 0; u < U; ++u) emit_compute(u, true);
         s << "      }\n";
-        // a kernel too large to track its lanes leaves every such chunk to
-        // the fix-up launch's comparison
-        s << "      u32 proven = " << (track ? "1u" : "0u") << ";\n";
-        if (track)
-            for (auto& kv : state_of) s << "      proven &= xs" << kv.second << ";\n";
-        s << "      if (!proven && lane == 0) {\n";
-        s << "        __hip_atomic_fetch_or(&A.unproven[c >> 5], 1u << (c & 31u), __ATOMIC_RELAXED, AGENT);\n";
-        s << "        __hip_atomic_fetch_add(&A.kstat[2], 1u, __ATOMIC_RELAXED, AGENT);\n";
-        s << "      }\n";
+        // a kernel too large to track its lanes proves only warm-ups that
+        // start at the instance's first row
+        s << "      proven = r_begin <= " << WR << "ull ? 1u : " << (track ? "1u" : "0u") << ";\n";
+        if (track) {
+            s << "      if (r_begin > " << WR << "ull) {\n";
+            for (auto& kv : state_of) s << "        proven &= xs" << kv.second << ";\n";
+            s << "      }\n";
+        }
         s << "    }\n";
-        // the state this chunk starts from (U[c]): speculative in the main
-        // launch, the predecessor's true end state in a recomputation
-        s << "    if (fix) {\n";
-        for (auto& kv : state_of)
-            s << "      if (lane == " << (kv.second % 64) << ") astore64(&A.st_u[(u64)c * " << NS << " + " << kv.second
-              << "u], st" << kv.second << ");\n";
-        s << "    } else {\n";
-        for (auto& kv : state_of)
-            s << "      if (lane == " << (kv.second % 64) << ") A.st_u[(u64)c * " << NS << " + " << kv.second << "u] = st"
-              << kv.second << ";\n";
-        s << "    }\n";
+        emit_tagged("spec");
+        for (int p = 0; p < NPOP; ++p) s << "    u32 pc" << p << " = 0u;\n";
+        // Pass 0 runs the chunk from the speculative state.  An unproven
+        // chunk then looks back: it waits for its predecessor's end state of
+        // this run (final once published) and, if that differs from the
+        // state it assumed, runs the chunk again from it (pass 1).  Chunks
+        // are owned by waves in increasing order and the hardware dispatches
+        // workgroups in order, so the lowest unfinished chunk never waits on
+        // an unfinished one; the wait is bounded all the same.
+        s << "    for (u32 pass = 0u;; ++pass) {\n";
+        for (int p = 0; p < NPOP; ++p) s << "    pc" << p << " = 0u;\n";
+    } else {
+        for (int p = 0; p < NPOP; ++p) s << "    u32 pc" << p << " = 0u;\n";
     }
-    for (int p = 0; p < NPOP; ++p) s << "    u32 pc" << p << " = 0u;\n";
     if (inter) {
         s << "    const u64 istride = (u64)D.n_chunks;\n";
         s << "    for (u64 r = cc; r < D.rows; r += istride * " << U << "ull) {\n";
@@ -775,50 +726,49 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     }
     for (int u = 0; u < U; ++u) emit_compute(u, false);
     s << "    }\n";  // row loop
+    if (has_state) {
+        s << "    if (pass == 0u && !proven) {\n";
+        s << "      if (lane == 0u) __hip_atomic_fetch_add(&kst[1], 1u, __ATOMIC_RELAXED, AGENT);\n";
+        s << "      const u64* const prev = A.st_t + (u64)(c - 1u) * " << NS2 << ";\n";
+        s << "      u64 w = spec;\n";
+        s << "      for (u32 it = 0u;; ++it) {\n";
+        s << "        const u64 x = lane < " << NS2 << " ? aload64((u64*)prev + lane) : ((u64)tag << 32);\n";
+        s << "        if (ballot((u32)(x >> 32) != tag) == 0ull) { w = x; break; }\n";
+        s << "        if (it >= A.spin_limit) {\n";
+        s << "          if (lane == 0u) {\n";
+        s << "            __hip_atomic_fetch_or(A.err, 1u, __ATOMIC_RELAXED, AGENT);\n";
+        s << "            __hip_atomic_fetch_add(&kst[2], 1u, __ATOMIC_RELAXED, AGENT);\n";
+        s << "          }\n";
+        s << "          break;\n";
+        s << "        }\n";
+        s << "        __builtin_amdgcn_s_sleep(2);\n";
+        s << "      }\n";
+        s << "      if (ballot(w != spec) != 0ull) {\n";
+        s << "        if (lane == 0u) __hip_atomic_fetch_add(&kst[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
+        for (auto& kv : state_of) {
+            const int i = kv.second;
+            s << "        st" << i << " = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)w, " << 2 * i + 1
+              << ") << 32) | (u64)(u32)__builtin_amdgcn_readlane((int)(u32)w, " << 2 * i << ");\n";
+        }
+        s << "        continue;\n";
+        s << "      }\n";
+        s << "    }\n";
+        s << "    break;\n";
+        s << "    }\n";  // pass loop
+        // the end state, final: this chunk's successor may be waiting for it
+        emit_tagged("fin");
+        s << "    if (lane < " << NS2 << ") astore64(&A.st_t[(u64)c * " << NS2 << " + lane], fin);\n";
+    }
 
     // popcounts: one atomic per chunk into slot c mod 2^spread_log
     for (int p = 0; p < NPOP; ++p) {
         s << "    {\n      const u32 tot = uni(wave_sum(pc" << p << "));\n";
-        s << "      if (lane == 0) {\n";
-        s << "        u64* acc = &bank[(((u64)ii * " << NPOP << "u + " << p
-          << "u) << A.spread_log) + (c & ((1u << A.spread_log) - 1u))];\n";
-        if (has_state) {
-            s << "        u32* part = &A.partial[(u64)c * " << NPOP << "u + " << p << "u];\n";
-            s << "        const u32 old = fix ? aload32(part) : 0u;\n";
-            s << "        if (tot != old) __hip_atomic_fetch_add(acc, (u64)tot - (u64)old, __ATOMIC_RELAXED, AGENT);\n";
-            s << "        if (fix) astore32(part, tot); else *part = tot;\n";
-        } else {
-            s << "        __hip_atomic_fetch_add(acc, (u64)tot, __ATOMIC_RELAXED, AGENT);\n";
-        }
-        s << "      }\n    }\n";
-    }
-    if (has_state) {
-        // end state T[c]
-        s << "    if (!fix) {\n";
-        for (auto& kv : state_of)
-            s << "      if (lane == " << (kv.second % 64) << ") A.st_t[(u64)c * " << NS << " + " << kv.second << "u] = st"
-              << kv.second << ";\n";
-        s << "      return;\n";
+        s << "      if (lane == 0) __hip_atomic_fetch_add(&bank[(((u64)ii * " << NPOP << "u + " << p
+          << "u) << A.spread_log) + (c & ((1u << A.spread_log) - 1u))], (u64)tot, __ATOMIC_RELAXED, AGENT);\n";
         s << "    }\n";
-        // the corrected end state stays in registers for the successor,
-        // which is checked next
-        for (auto& kv : state_of)
-            s << "    if (lane == " << (kv.second % 64) << ") astore64(&A.st_t[(u64)c * " << NS << " + " << kv.second
-              << "u], st" << kv.second << ");\n";
-        s << "    casc = true;\n";
-        s << "    last_c = c;\n";
-        s << "    ++n_re;\n";
-        s << "  }\n";  // job loop
-        s << "fix_done:\n";
-        s << "  if (lane == 0) {\n";
-        s << "    astore32(&A.kstat[0], use_bits ? n_mis : n_re);\n";
-        s << "    astore32(&A.kstat[1], n_re);\n";
-        s << "    if (A.last) astore32(&A.ctl[0], aload32(&A.ctl[0]) + 1u);\n";
-        s << "  }\n";
-    } else {
-        s << "    return;\n";
-        s << "  }\n";  // job loop (one pass)
     }
+    emit_run_epilogue(s);
+    s << "  }\n";  // chunk
     s << "}\n";
     ks.source = s.str();
 }
@@ -856,7 +806,7 @@ std::string PlanOptions::key() const {
     std::ostringstream s;
     s << chunk_rows << "," << warmup_rows << "," << waves_per_block << "," << rows_per_step << "," << max_ops << ","
       << max_live << "," << occupancy << "," << words_per_lane << "," << store_order << "," << nontemporal << ","
-      << interleave;
+      << interleave << "," << persistent;
     return s.str();
 }
 

@@ -14,32 +14,34 @@
 //     is the next row's carry in;
 //   * those scalars (shift neighbours and carries) are the kernel's state.  A
 //     chunk other than the first starts `warmup_rows` rows early with a zero
-//     state (speculation), records the state it reaches at its first row (U)
-//     and the state at its end (T).  Chunk c was exact iff U[c] == T[c-1].
-//     During the warm-up the wave also tracks, per value, the lowest lane
-//     from which it is certainly exact, and per state word whether it is:
-//     a wrong start state damages only the low lanes of a row, an advance's
-//     state is its operand's lane 63, and a carry is settled by any lane that
-//     generates or kills it above the damage.  A chunk whose warm-up proves
-//     every state exact is final; others are marked in a bitmap.
-//     The main launch has no cross-wave dependency at all (no tickets, no
-//     waits, no flags): wave w of the grid owns chunk w.  A second launch of
-//     the same kernel (one workgroup, mode 1) ends at once when no chunk was
-//     marked; otherwise it compares U[c] with T[c-1] for every marked chunk in
-//     parallel, marks the mismatches in an LDS bitmap, and one wave
-//     recomputes them in order from the predecessor's true end state,
-//     following a changed end state into the next chunk.  The result is exact
-//     for every input; speculation only decides how much work is redone;
+//     state (speculation).  During the warm-up the wave also tracks, per
+//     value, the lowest lane from which it is certainly exact, and per state
+//     word whether it is: a wrong start state damages only the low lanes of a
+//     row, an advance's state is its operand's lane 63, and a carry is
+//     settled by any lane that generates or kills it above the damage.  A
+//     chunk whose warm-up proves every state exact (or starts at the
+//     instance's first row) is final.  An unproven chunk runs its rows from
+//     the speculative state, then looks back: it waits for its
+//     predecessor's end state of this run and, when that differs from the
+//     state it assumed, runs its rows again from it.  Every chunk publishes
+//     its final end state as 64-bit words that each carry one 32-bit half
+//     under the run's tag, so a reader checks the tags and needs no fence.
+//     Wave w of the grid owns chunk w; workgroups are dispatched in order,
+//     so the lowest unfinished chunk never waits for an unfinished one, and
+//     each wait is bounded (a wait that reaches the bound sets the plan's
+//     error word).  One launch per kernel; the result is exact for every
+//     input, and speculation only decides how much work is redone;
 //   * an advance of a value computed by an earlier kernel (any k) and `>>` are
 //     direct loads at the shifted word offset, needing no state;
 //   * popcounts accumulate per lane (v_bcnt), are reduced per wave and added
 //     with one atomic per chunk into one of 2^spread_log slots (chunk mod
 //     2^spread_log, up to kMaxPopSpread slots, about one per 4 chunks of the
 //     longest instance), so no address serializes more than a few atomics;
-//     the host sums the slots.  Slots live in two banks: a run adds
-//     into bank (epoch & 1) and clears the other for the next run, and the
-//     run's last launch advances the epoch, so no separate clearing launch is
-//     needed.
+//     the host sums the slots.  Slots live in two banks: a run adds into
+//     bank (run & 1), where `run` counts the plan's completed runs over the
+//     chunk, and clears the other for the next run; the plan's last kernel
+//     advances each chunk's run number, so no separate clearing or epoch
+//     launch is needed.
 #pragma once
 
 #include <cstdint>
@@ -82,6 +84,7 @@ struct PlanOptions {
     int max_ops = 4000;
     int max_live = 96;
     int occupancy = 0;               // waves per SIMD the register budget allows; 0 = from an estimate
+    bool persistent = false;         // grid of resident waves, each striding over the chunks
     std::string key() const;
 };
 
@@ -146,34 +149,32 @@ struct InstDesc {
 
 // Most popcount accumulator slots per (instance, popcount).
 constexpr int kMaxPopSpreadLog = 8;
-// LDS bitmap of the fix-up launch: chunks beyond it are checked one by one.
-constexpr uint32_t kFixBitmapWords = 8192;
-// Threads of the fix-up launch.
-constexpr int kFixThreads = 256;
+// Polls of a look-back wait before it gives up and flags the plan (each poll
+// is a memory round trip and a short sleep: seconds in all).
+constexpr uint32_t kSpinLimit = 1u << 22;
 
-// ctl block of a plan: [0] epoch (runs completed); per kernel k, at
-// 4 + 4k: [0] chunks whose speculation failed in the last run, [1] chunks
-// recomputed, [2] chunks the current main launch could not prove exact
-// (cleared by the fix-up launch), [3] that count for the last run.
+// ctl block of a plan: [0] error word (bit 0: a look-back wait reached its
+// bound); per kernel k, at 4 + 8k, two banks of four counters, bank (run & 1)
+// counting the run: [0] chunks run again from their predecessor's end state,
+// [1] chunks whose warm-up did not prove their start state, [2] look-back
+// waits that reached the bound.
 struct KArgs {
     const uint64_t* const* in;        // [inst][n_in]
     uint64_t* const* out;             // [inst][n_out]
     unsigned long long* pop_acc;      // bank 0: [inst][n_pop_k][2^spread_log]; bank 1 at + pop_stride
     const InstDesc* inst;
     const uint32_t* chunk_inst;       // [n_chunks] instance of each chunk
-    unsigned long long* st_u;         // [n_chunks][n_state]
-    unsigned long long* st_t;         // [n_chunks][n_state]
-    unsigned* partial;                // [n_chunks][n_pop_k] (stateful kernels)
-    unsigned* ctl;                    // the plan's ctl block
-    unsigned* kstat;                  // this kernel's stats in it
-    unsigned* unproven;               // [ceil(n_chunks/32)] chunks whose warm-up left a state unproven
+    unsigned long long* st_t;         // [n_chunks][2 * n_state] tagged end-state halves
+    unsigned* runs;                   // [n_chunks] completed runs of the plan over each chunk
+    unsigned* kstat;                  // this kernel's counters in the ctl block
+    unsigned* err;                    // the plan's error word
     uint64_t pop_stride;              // words between the two banks
     uint32_t n_chunks;
     uint32_t n_inst;
-    uint32_t mode;                    // 0 main launch, 1 fix-up (one workgroup)
-    uint32_t last;                    // 1: this launch ends the run (advances the epoch)
+    uint32_t last;                    // 1: the plan's last kernel (advances the run numbers)
     uint32_t spread_log;              // log2 of the popcount slots per (instance, popcount)
-    uint32_t pad_;
+    uint32_t spin_limit;              // polls of a look-back wait before it gives up
+    uint32_t grid_waves;              // waves in the grid (a persistent kernel strides by it)
 };
 
 }  // namespace bsz

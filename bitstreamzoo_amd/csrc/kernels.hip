@@ -288,10 +288,6 @@ __global__ __launch_bounds__(256) void k_tail(uint64_t* dst, uint64_t k, uint64_
 
 // Ends a fused plan run whose last kernel has no fix-up launch: the run
 // counter selects the popcount bank the next run adds into.
-__global__ __launch_bounds__(64) void k_plan_epoch(unsigned* ctl) {
-    if (threadIdx.x == 0) ctl[0] += 1u;
-}
-
 __global__ __launch_bounds__(256) void k_copy_words(uint64_t* __restrict__ dst, const uint64_t* __restrict__ src,
                                                     uint64_t n_pairs) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -531,11 +527,6 @@ void upload_stream(hipStream_t s, uint64_t* dst, const uint64_t* host, uint64_t 
     const uint64_t total = g.alloc_words();
     // opmask is lastmask for clean values and all ones under bsim padding
     k_tail<<<std::max(1u, blocks_for(total - k, 256)), 256, 0, s>>>(dst, k, total, g.lw, g.opmask);
-    check_launch();
-}
-
-void launch_plan_epoch(hipStream_t s, unsigned* ctl) {
-    k_plan_epoch<<<1, 64, 0, s>>>(ctl);
     check_launch();
 }
 

@@ -133,7 +133,25 @@ def test_chained_adds_and_advances(tile, kind):
     got, mism = run_plan(CARRY_SRC, [case], tile, names)
     assert_same(got[0], ora, bl, set(names[1]), f"carry {kind} {tile}")
     if kind == "chain" and tile["chunk_rows"] <= 4 and tile.get("max_ops", 100) > 3:
-        assert sum(mism) > 0  # the speculation was wrong and the fix-up ran
+        assert sum(mism) > 0  # the speculation was wrong and chunks ran again after their look-back
+
+
+def test_lookback_wait_bound_reports_an_error():
+    """A look-back wait is bounded: with a bound of one poll, a chunk whose
+    predecessor has not published its end state yet gives up at once, and
+    the run reports an error instead of outputs.  (One carry rippling
+    through thousands of 1-row chunks: no warm-up can prove a start state,
+    and the end states are published one after another.)  The same plan
+    shape with the library's bound is exact."""
+    bl = 64 * 64 * 6000 + 13
+    case = carry_case(bl, "chain", 5)
+    ora, names = oracle_outs(CARRY_SRC, case)
+    tile = {"chunk_rows": 1, "rows_per_step": 1, "warmup_rows": 1}
+    with pytest.raises(RuntimeError, match="look-back wait"):
+        run_plan(CARRY_SRC, [case], dict(tile, spin_limit=0), names)
+    got, mism = run_plan(CARRY_SRC, [case], tile, names)
+    assert_same(got[0], ora, bl, set(names[1]), "carry chain, default bound")
+    assert sum(mism) > 0
 
 
 def test_batched_shift_and_instances():
