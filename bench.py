@@ -197,14 +197,45 @@ def bench_matchstar(ctx, args):
         log(f"matchstar: per-op {perop:.4f} ms/step")
     info = B.plans[0].info()
     B.free()
+    floor_ms, floor_opts = one_pass_floor(ctx, args, residents, ws[0].cases[0].bitlength)
+    log(f"matchstar: one-launch floor (r = M & C over the same inputs) {floor_ms:.4f} ms/step {floor_opts}")
     for r in residents:
         r.free()
     out = {"ms_per_step": ms, "bit_ops": ws[0].bit_ops,
            "alg_bytes": ws[0].alg_bytes, "plan": info, "input_sets": rot, "options": opts, "tune": tune,
-           "unproven_chunks": unproven}
+           "unproven_chunks": unproven, "floor_ms": floor_ms, "floor_options": floor_opts}
     if perop is not None:
         out["perop_ms_per_step"] = perop
     return ws[0], out
+
+
+FLOOR_SRC = """input stream M
+input stream C
+output stream r
+r = M & C
+"""
+
+
+def one_pass_floor(ctx, args, residents, bitlength):
+    """ms per step of the cheapest one-launch pass that moves MatchStar's
+    bytes: r = M & C over the same rotating input sets (two streams read, one
+    written, no carry, no warm-up).  A program of this size cannot run
+    faster than one such launch; the best of a few tile shapes is kept."""
+    prog = api.Program(FLOOR_SRC)
+    best = None
+    for opts in ({"chunk_rows": 4, "rows_per_step": 4}, {"chunk_rows": 8, "rows_per_step": 8},
+                 {"chunk_rows": 16, "rows_per_step": 8}, {"chunk_rows": 16, "rows_per_step": 4, "interleave": 1}):
+        plans = []
+        for r in residents:
+            p = api.Plan(ctx, prog, tuning.shape_case(r.cases), opts)
+            r.bind(p)
+            plans.append(p)
+        ms = time_plans(ctx, plans, args.steps, args.warmup, api.FUSED)
+        for p in plans:
+            p.close()
+        if best is None or ms < best[0]:
+            best = (ms, opts)
+    return best
 
 
 def bench_shift_and(ctx, args):
@@ -536,6 +567,11 @@ def main():
             ex[name]["speedup_fused_over_perop"] = round(rr["perop_ms_per_step"] / m, 2)
         if "unproven_chunks" in rr:
             ex[name]["unproven_chunks"] = rr["unproven_chunks"]
+        if "floor_ms" in rr:
+            # the same bytes in the cheapest single launch (r = M & C)
+            ex[name]["one_launch_floor_ms"] = round(rr["floor_ms"], 5)
+            ex[name]["one_launch_floor_hbm_frac"] = round(rr["alg_bytes"] / (rr["floor_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+            ex[name]["frac_of_floor"] = round(rr["floor_ms"] / m, 4)
     if ex:
         line["other_configs"] = ex
     if learned is not None:

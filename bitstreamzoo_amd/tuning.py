@@ -253,9 +253,17 @@ class TuneResult:
     same_as: list = field(default_factory=list)   # (candidate, earlier candidate with the same kernels)
     seconds: float = 0.0
 
+    # Schedules within this fraction of the fastest count as tied: timing
+    # noise between runs is about that size.
+    TIE = 0.01
+
     @property
     def best(self) -> Trial:
-        return min(self.trials, key=lambda t: t.ms)
+        """The fastest schedule; among schedules tied with it, the first in
+        candidate order, so repeated tunings of one workload agree on the
+        pick whenever the measurements only differ by noise."""
+        fastest = min(t.ms for t in self.trials)
+        return next(t for t in self.trials if t.ms <= fastest * (1.0 + self.TIE))
 
     def as_json(self) -> dict:
         b = self.best

@@ -24,7 +24,9 @@ def test_autotune_adder_small():
     assert [t.options for t in tr.trials] == space and not tr.skipped
     assert tr.trials[1].info["kernels"][0]["rows_per_step"] == 1
     assert all(k["agprs"] == 0 and k["scratch_bytes"] == 0 for t in tr.trials for k in t.info["kernels"])
-    assert tr.best.ms == min(t.ms for t in tr.trials)
+    fastest = min(t.ms for t in tr.trials)
+    assert tr.best.ms <= fastest * (1 + tuning.TuneResult.TIE)
+    assert tr.best is next(t for t in tr.trials if t.ms <= fastest * (1 + tuning.TuneResult.TIE))
     assert all(t.ms > 0 for t in tr.trials)
     res.free()
 
