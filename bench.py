@@ -291,8 +291,8 @@ def bench_regex_dag(ctx, args):
     p = dag_gen.generate(nodes, seed=4)
     case = dag_gen.text_case(nbytes, seed=11)
     prog = api.Program(p.source)
-    # 4 warm-up rows: 9 % faster than 1 here (fewer chunks start from a wrong
-    # carry, so fewer are redone in order by the fix-up launch)
+    # 4 warm-up rows: faster than 1 here (fewer chunks start from a wrong
+    # carry, so fewer are run again after their look-back)
     opts = json.loads(args.schedule) if args.schedule else {"warmup_rows": 4}
     t0 = time.perf_counter()
     tuning.precompile(p.source, [case], [opts], processes=16, shards=64, log=lambda m: log(f"regex_dag: {m}"))
@@ -315,10 +315,9 @@ def bench_regex_dag(ctx, args):
     if bad:
         raise SystemExit(f"regex_dag: {len(bad)} fused outputs differ from the per-op executor's")
     del fused
-    launches = len(ks) + sum(1 for k in ks if k["state_words"] > 0) + (0 if ks[-1]["state_words"] else 1)
-    ms = time_plans(ctx, [plan], args.steps, args.warmup, api.FUSED, launches_per_run=launches)
+    ms = time_plans(ctx, [plan], args.steps, args.warmup, api.FUSED, launches_per_run=len(ks))
     mism = [plan.mismatches(k) for k in range(len(ks))]
-    log(f"regex_dag: fused {ms:.1f} ms/step; {sum(mism)} chunks redone by fix-up launches "
+    log(f"regex_dag: fused {ms:.1f} ms/step; {sum(mism)} chunks ran again after their look-back "
         f"({sum(1 for m in mism if m)} kernels)")
     perop = time_plans(ctx, [plan], 1, 0, api.PEROP, launches_per_run=info["graph_nodes"])
     log(f"regex_dag: per-op {perop:.1f} ms/step")
@@ -330,7 +329,7 @@ def bench_regex_dag(ctx, args):
     return w, {"ms_per_step": ms, "perop_ms_per_step": perop, "bit_ops": w.bit_ops, "alg_bytes": alg,
                "planned_bytes": planned, "lane_ops": 2 * p.n_dsl_ops * nw, "plan": info, "options": opts,
                "compile_s": compile_s, "dsl_ops": p.n_dsl_ops, "outputs_checked": len(p.outputs),
-               "fixup_chunks": int(sum(mism)), "chunks_per_kernel": info["n_chunks"]}
+               "rerun_chunks": int(sum(mism)), "chunks_per_kernel": info["n_chunks"]}
 
 
 def bench_learned(ctx, args):
@@ -354,6 +353,92 @@ def bench_learned(ctx, args):
             "time_ratio_pct": [round(100 * r.model_s / r.tune_s, 2) for r in rows],
             "mape_pct": [round(100 * r.mape, 1) for r in rows],
             "pick_vs_default_speedup": [round(r.default_ms / r.pick_ms, 3) for r in rows]}
+
+
+def bench_mixed(ctx, args):
+    """A mixed batch of generated regex-style DAG programs, each over its own
+    text, all run per step as one captured hipGraph of every plan's launches.
+    Each program's schedule is the learned model's top prediction over 64
+    sampled candidates (no trial runs); every program's outputs are checked
+    against the same program under the default schedule, and the measured
+    autotuner re-tunes a few programs over 16 of the candidates as judge.
+    Sizes are reduced from 1k-300k ops over 2^16-2^26 bytes to
+    `--mixed-nodes` and `--mixed-log2-bytes`, so that compiling 256 programs
+    stays within the benchmark's minutes."""
+    from bitstreamzoo_amd import costmodel, schedsearch
+    n = args.mixed_programs
+    lo, hi = (int(x) for x in args.mixed_nodes.split(","))
+    blo, bhi = (float(x) for x in args.mixed_log2_bytes.split(","))
+    specs = schedsearch.random_specs(n, 55, nodes=(lo, hi), log2_bytes=(blo, bhi))
+    built = [sp.build() for sp in specs]
+    X, ms_train = schedsearch.load_training_data()
+    model = costmodel.CostModel(ctx, seed=0)
+    model.fit(X, ms_train)
+    cands = schedsearch.sample_space(64, 12345)
+    t0 = time.perf_counter()
+    picks = []
+    for p, case in built:
+        F = costmodel.features_batch(api.Program(p.source), tuning.shape_case([case]), cands,
+                                     max(1, (case.bitlength + 63) // 64), 1)
+        picks.append(cands[int(np.argmin(model.predict_ms(F)))])
+    pick_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    tuning.precompile_many([(p.source, [case], o) for (p, case), o in zip(built, picks)] +
+                           [(p.source, [case], {}) for p, case in built], processes=16,
+                           log=lambda m: log(f"mixed: {m}"))
+    compile_s = time.perf_counter() - t1
+    log(f"mixed: {n} programs, picks in {pick_s:.1f} s, compiled in {compile_s:.1f} s")
+    residents, plans, defaults = [], [], []
+    bit_ops = alg = launches = 0
+    try:
+        for (p, case), o in zip(built, picks):
+            r = tuning.Resident(ctx, [case])
+            residents.append(r)
+            prog = api.Program(p.source)
+            pl = api.Plan(ctx, prog, tuning.shape_case([case]), o)
+            r.bind(pl)
+            plans.append(pl)
+            d = api.Plan(ctx, prog, tuning.shape_case([case]), {})
+            r.bind(d)
+            defaults.append(d)
+            bit_ops += p.n_dsl_ops * case.bitlength
+            alg += 8 * ((case.bitlength + 63) // 64) * (8 + len(p.outputs))
+            launches += len(pl.info()["kernels"])
+        for pl, d, (p, case) in zip(plans, defaults, built):
+            pl.run()
+            d.run()
+        ctx.sync()
+        for pl, d, (p, case) in zip(plans, defaults, built):
+            nw = (case.bitlength + 63) // 64
+            for o in p.outputs:
+                if not np.array_equal(pl.download_output(0, o, nw), d.download_output(0, o, nw)):
+                    raise SystemExit(f"mixed: a program's output {o} differs between two schedules")
+        ms = time_plans(ctx, plans, args.steps, args.warmup, api.FUSED) * len(plans)
+        default_ms = time_plans(ctx, defaults, args.steps, args.warmup, api.FUSED) * len(defaults)
+    finally:
+        for pl in plans + defaults:
+            pl.close()
+        for r in residents:
+            r.free()
+    log(f"mixed: batch {ms:.3f} ms/step with the model's schedules, {default_ms:.3f} ms with the defaults "
+        f"({launches} launches per step)")
+    judged = []
+    rng = np.random.Generator(np.random.PCG64(5))
+    sub = [cands[i] for i in sorted(rng.choice(len(cands), size=16, replace=False).tolist())]
+    for i in sorted(rng.choice(n, size=min(n, args.mixed_judged), replace=False).tolist()):
+        p, case = built[i]
+        trial = [picks[i]] + [c for c in sub if c != picks[i]]
+        m = schedsearch.measure(ctx, p.source, case, trial, processes=16)
+        if picks[i] not in m.options:
+            continue
+        pick_ms = float(m.ms[m.options.index(picks[i])])
+        judged.append({"program": i, "nodes": specs[i].nodes, "bytes": specs[i].n_bytes,
+                       "pick_ms": round(pick_ms, 4), "autotuned_best_ms": round(float(min(m.ms)), 4),
+                       "gap_pct": round(100 * (pick_ms / float(min(m.ms)) - 1), 2)})
+        log(f"mixed: judged program {i}: pick {pick_ms:.3f} ms, autotuned best {min(m.ms):.3f} ms")
+    return {"programs": n, "nodes": [lo, hi], "log2_bytes": [blo, bhi], "ms_per_step": ms,
+            "default_ms_per_step": default_ms, "launches_per_step": launches, "bit_ops": bit_ops,
+            "alg_bytes": alg, "pick_s": round(pick_s, 2), "compile_s": round(compile_s, 1), "judged": judged}
 
 
 def copy_bandwidth(ctx) -> float:
@@ -455,7 +540,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="adder32", choices=["adder32", "matchstar", "shift_and", "regex_dag"])
+    ap.add_argument("--workload", default="adder32", choices=["adder32", "matchstar", "shift_and", "regex_dag", "mixed"])
+    ap.add_argument("--no-dag", action="store_true", help="leave the regex DAG and the mixed batch out of the default run")
     ap.add_argument("--dag-nodes", type=int, default=200_000, help="generator nodes of the regex_dag workload")
     ap.add_argument("--bitlength", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -463,6 +549,10 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner; use fixed default schedules")
     ap.add_argument("--schedule", default="", help="JSON options for the measured workload's plan (skips tuning)")
+    ap.add_argument("--mixed-programs", type=int, default=256, help="programs of the mixed batch (0 skips it)")
+    ap.add_argument("--mixed-nodes", default="300,3000", help="generator nodes of the mixed batch, low,high")
+    ap.add_argument("--mixed-log2-bytes", default="16,20", help="text bytes (log2) of the mixed batch, low,high")
+    ap.add_argument("--mixed-judged", type=int, default=2, help="mixed-batch programs re-tuned by the autotuner")
     ap.add_argument("--learned-programs", type=int, default=1,
                     help="held-out programs for the learned-vs-autotuned comparison (0 skips it)")
     args = ap.parse_args()
@@ -480,6 +570,15 @@ def main():
         w, r = bench_matchstar(ctx, args)
     elif args.workload == "regex_dag":
         w, r = bench_regex_dag(ctx, args)
+    elif args.workload == "mixed":
+        r = bench_mixed(ctx, args)
+        m = r["ms_per_step"]
+        print(json.dumps({"metric": METRIC, "workload": "mixed", "ms_per_step": round(m, 5),
+                          "value": round(r["bit_ops"] / (m * 1e-3) / 1e9, 1), "unit": "Gbit-ops/s",
+                          "hbm_frac": round(r["alg_bytes"] / (m * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                          **{k: v for k, v in r.items() if k != "ms_per_step"}}), flush=True)
+        ctx.close()
+        return 0
     else:
         w, r = bench_shift_and(ctx, args)
     results[args.workload] = r
@@ -492,6 +591,18 @@ def main():
                 raise
             except Exception as e:  # report, do not hide
                 extra[name] = {"error": str(e)}
+        if not args.no_dag:
+            # the 200k-node DAG at its default schedule, 3 timed steps
+            dag_args = argparse.Namespace(**dict(vars(args), steps=3, warmup=1, schedule=""))
+            for name, fn in (("regex_dag", lambda c, a: bench_regex_dag(c, a)[1]), ("mixed", bench_mixed)):
+                if name == "mixed" and args.mixed_programs <= 0:
+                    continue
+                try:
+                    extra[name] = fn(ctx, dag_args)
+                except SystemExit:
+                    raise
+                except Exception as e:  # report, do not hide
+                    extra[name] = {"error": str(e)}
     learned = None
     if not args.no_extra and args.workload == "adder32" and args.learned_programs > 0:
         try:
@@ -539,7 +650,7 @@ def main():
         line["roofline"]["planned_bytes"] = r["planned_bytes"]
         line["roofline"]["planned_frac"] = round(r["planned_bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
         line["compile_s"] = round(r["compile_s"], 1)
-        line["fixup_chunks"] = r["fixup_chunks"]
+        line["rerun_chunks"] = r["rerun_chunks"]
         line["outputs_checked_against_perop"] = r["outputs_checked"]
     # VALU side of the roofline: 2 32-bit lane-ops per 64-bit word per DSL op
     lane_ops = r.get("lane_ops", 2 * r["bit_ops"] // 64)
@@ -561,12 +672,24 @@ def main():
         ex[name] = {"ms_per_step": round(m, 5), "value": round(rr["bit_ops"] / (m * 1e-3) / 1e9, 1),
                     "GBps": round(rr["alg_bytes"] / (m * 1e-3) / 1e9, 1),
                     "hbm_frac": round(rr["alg_bytes"] / (m * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-                    "schedule": rr["options"], "tuning_s": rr["tune"]["tuning_s"] if rr.get("tune") else None}
+                    "schedule": rr.get("options"), "tuning_s": rr["tune"]["tuning_s"] if rr.get("tune") else None}
         if "perop_ms_per_step" in rr:
             ex[name]["perop_ms_per_step"] = round(rr["perop_ms_per_step"], 5)
             ex[name]["speedup_fused_over_perop"] = round(rr["perop_ms_per_step"] / m, 2)
         if "unproven_chunks" in rr:
             ex[name]["unproven_chunks"] = rr["unproven_chunks"]
+        if name == "regex_dag":
+            ex[name]["planned_bytes"] = rr["planned_bytes"]
+            ex[name]["planned_hbm_frac"] = round(rr["planned_bytes"] / (m * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+            ex[name]["valu_frac"] = round(rr["lane_ops"] / VALU_PEAK_LANE_OPS / (m * 1e-3), 4)
+            ex[name]["kernels"] = len(rr["plan"]["kernels"])
+            ex[name]["rerun_chunks"] = rr["rerun_chunks"]
+            ex[name]["compile_s"] = round(rr["compile_s"], 1)
+            ex[name]["outputs_checked_against_perop"] = rr["outputs_checked"]
+        if name == "mixed":
+            for k in ("programs", "nodes", "log2_bytes", "default_ms_per_step", "launches_per_step", "pick_s",
+                      "compile_s", "judged"):
+                ex[name][k] = rr[k]
         if "floor_ms" in rr:
             # the same bytes in the cheapest single launch (r = M & C)
             ex[name]["one_launch_floor_ms"] = round(rr["floor_ms"], 5)

@@ -79,6 +79,8 @@ FusedOptions FusedOptions::from_json_obj(const Json& j) {
     get("max_live", o.max_live);
     get("occupancy", o.occupancy);
     get("persistent", o.persistent);
+    get("agprs", o.agprs);
+    get("lazy_loads", o.lazy_loads);
     get("compile", o.compile);
     get("compile_shard", o.compile_shard);
     get("compile_shards", o.compile_shards);
@@ -102,6 +104,8 @@ PlanOptions to_plan_options(const FusedOptions& f) {
     p.max_live = std::max(4, f.max_live);
     p.occupancy = f.occupancy <= 0 ? 0 : std::min(8, f.occupancy);
     p.persistent = f.persistent != 0;
+    p.agprs = f.agprs != 0 && p.occupancy == 1;
+    p.lazy_loads = f.lazy_loads != 0;
     return p;
 }
 
@@ -233,13 +237,18 @@ KernelRegs kernel_regs(const std::vector<char>& code) {
 // refused rather than run.  (Two wide random-DAG partitions whose kernels
 // overflowed also gave 64-bit values with correct low and wrong high halves;
 // that is the signature of the OR miscompile the generated code now guards
-// against with opq(), see fused_codegen.cpp.)
-bool regs_clean(const KernelRegs& r) { return r.vgprs > 0 && r.agprs == 0 && r.scratch == 0; }
+// against with opq(), see fused_codegen.cpp.)  A schedule that asks for the
+// AGPRs ({"occupancy": 1, "agprs": 1}) lets a kernel keep values there, the
+// second half of the 512 registers a wave has at occupancy 1; scratch is
+// never allowed.
+bool regs_clean(const KernelRegs& r, bool allow_agprs) {
+    return r.vgprs > 0 && r.agprs >= 0 && (allow_agprs || r.agprs == 0) && r.scratch == 0;
+}
 
 void refuse_unclean(const PlanSpec& p, const std::vector<KernelRegs>& regs) {
     for (size_t k = 0; k < regs.size() && k < p.kernels.size(); ++k) {
         const KernelRegs& r = regs[k];
-        if (regs_clean(r)) continue;
+        if (regs_clean(r, p.opt.agprs)) continue;
         std::ostringstream m;
         m << "schedule refused: generated kernel " << p.kernels[k].name << " (" << p.kernels[k].nodes.size()
           << " nodes, " << p.kernels[k].inputs.size() << " inputs) ";
@@ -338,7 +347,7 @@ public:
         for (int round = 0; round < 12; ++round) {
             bool changed = false;
             for (size_t k = 0; k < p.kernels.size(); ++k)
-                if (!regs_clean(regs[k]) && shrink_kernel(p, k)) changed = true;
+                if (!regs_clean(regs[k], p.opt.agprs) && shrink_kernel(p, k)) changed = true;
             if (!changed) break;
             compile_all(p, compile_ms);
             regs = regs_of(p);
@@ -970,7 +979,7 @@ public:
         std::ostringstream o;
         o << "{\"executor\":\"fused\",\"flushes\":" << flushes_ << ",\"plans\":" << plans_.size()
           << ",\"kernels\":" << kernels_run_ << ",\"compile_ms\":" << compile_ms_
-          << ",\"fixup_chunks\":" << fixups_ << "}";
+          << ",\"rerun_chunks\":" << reruns_ << "}";
         return o.str();
     }
 
@@ -1020,7 +1029,7 @@ private:
     std::pair<std::string, int64_t> pending_input_{"", -1};
     std::unordered_map<std::string, CachedPlan> plans_;
 
-    int64_t launches_ = 0, launches0_ = 0, flushes_ = 0, kernels_run_ = 0, fixups_ = 0;
+    int64_t launches_ = 0, launches0_ = 0, flushes_ = 0, kernels_run_ = 0, reruns_ = 0;
     double compile_ms_ = 0;
 
     int32_t add_node(const GNode& n, const SV& v) {
@@ -1155,7 +1164,7 @@ void FusedEngine::flush() {
         }
         if (fopt_.verbose) {
             for (size_t k = 0; k < cp.spec->kernels.size(); ++k)
-                if (cp.spec->kernels[k].n_state) fixups_ += rt.mismatches(s_, k);
+                if (cp.spec->kernels[k].n_state) reruns_ += rt.mismatches(s_, k);
         }
         for (size_t i = 0; i < out_vals.size(); ++i) {
             out_vals[i]->buf = out_bufs[i];
@@ -1611,6 +1620,7 @@ std::string FusedPlan::info_json() const {
       << ",\"waves_per_block\":" << I.spec->opt.waves_per_block << ",\"rows_per_step\":" << I.spec->opt.rows_per_step
       << ",\"max_ops\":" << I.spec->opt.max_ops << ",\"max_live\":" << I.spec->opt.max_live
       << ",\"occupancy\":" << I.spec->opt.occupancy << ",\"persistent\":" << (I.spec->opt.persistent ? 1 : 0)
+      << ",\"agprs\":" << (I.spec->opt.agprs ? 1 : 0)
       << ",\"n_chunks\":" << (I.rt ? I.rt->n_chunks : 0) << ",\"device_bytes\":" << (I.rt ? I.rt->device_bytes() : 0)
       << "}";
     return o.str();

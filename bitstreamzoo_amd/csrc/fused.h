@@ -23,6 +23,9 @@ struct FusedOptions {
     int max_live = 96;          // live 64-bit values per lane per kernel
     int occupancy = 0;          // waves per SIMD the register budget allows (512/occupancy VGPRs); 0 = auto
     int persistent = 0;         // 1: each kernel's grid holds only resident waves, each striding over the chunks
+    int agprs = 0;              // 1 (with occupancy 1): kernels may keep values in AGPRs, 512 registers per wave
+    int lazy_loads = 0;         // 1: each input word is loaded at its first use in a row (partitions count it
+                                //   as a value from there to its last use) instead of at the row's start
     std::string cache_dir;      // code-object cache ("" = $BSZ_CACHE_DIR or ~/.cache/bsz_hip)
     int compile = 1;            // compile-only: 0 partitions and generates without running hipRTC
     int compile_shard = 0;      // compile-only: compile the kernels k with k % compile_shards == compile_shard

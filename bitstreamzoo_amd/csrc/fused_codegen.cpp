@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <map>
+#include <queue>
 #include <set>
 #include <sstream>
 #include <stdexcept>
@@ -180,16 +181,19 @@ std::string hex64(uint64_t v) {
 }
 
 // State words (carries, advance neighbours) one kernel may carry from row to
-// row; they live in scalar registers, spilled to VGPR lanes when many.
-constexpr int kMaxStateWords = 32;
+// row; they live in scalar registers, spilled to VGPR lanes when many.  A
+// kernel with the AGPRs has the registers for more of them.
+int max_state_words(const PlanOptions& opt) { return opt.agprs ? 128 : 32; }
 // Kernels of at most this many nodes prove their chunks exact in the warm-up.
 constexpr int kTrackNodes = 128;
 
 // Registers a generated kernel may use: the 256 architectural VGPRs at most
 // (values kept in AGPRs or scratch cost extra moves on every row), fewer at
-// a higher requested occupancy.
+// a higher requested occupancy; at occupancy 1 with AGPRs allowed, all 512
+// registers of the wave (the compiler moves values between the two files).
 int kernel_registers(const PlanOptions& opt) {
     const int occ = opt.occupancy > 0 ? opt.occupancy : 1;
+    if (opt.agprs && occ == 1) return 512;
     return std::min(256, 512 / occ);
 }
 
@@ -248,8 +252,8 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     const bool has_state = n_state > 0;
     // Lane-exactness tracking in warm-up rows costs a scalar per node; large
     // kernels skip it (their chunks after the first warm-up rows all look
-    // back at their predecessor's end state).
-    const bool track = has_state && (int)ks.nodes.size() <= kTrackNodes;
+    // back at their predecessor's end state), unless they have the AGPRs.
+    const bool track = has_state && ((int)ks.nodes.size() <= kTrackNodes || opt.agprs);
 
     // which graph nodes need a plain value of an input (loaded per row)
     std::set<int32_t> plain_inputs;
@@ -496,6 +500,62 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
 
     // Loads of U rows starting at row r (row u: `r + u`, or `r + u * istride`
     // when interleaved); row u exists while its index is below `limit`.
+    // Row u's words of the kernel's inputs.  `guard`: the row may not exist
+    // (the word is 0 then); inside a row's `if (ok)` block it does.
+    auto emit_x = [&](int u, int32_t a, bool guard) {
+        const std::string rb = "urow(in" + std::to_string(input_index.at(a)) + " + ru" + std::to_string(u) + " * 64ull)";
+        s << "      const u64 x" << a << "_" << u << " = ";
+        if (guard) s << "ok" << u << " ? lane_ld(" << rb << ", off8) : 0ull;\n";
+        else s << "lane_ld(" << rb << ", off8);\n";
+    };
+    auto emit_z = [&](int u, const std::pair<int32_t, long long>& ow, bool guard) {
+        const std::string d = ow.second < 0 ? " - " + std::to_string(-ow.second) + "ll" : " + " + std::to_string(ow.second) + "ll";
+        s << "      const u64 z" << ow.first << "_" << (ow.second < 0 ? "m" : "p") << std::llabs(ow.second) << "_" << u << " = ";
+        const std::string e = "ldr(urow(in" + std::to_string(input_index.at(ow.first)) + " + ru" + std::to_string(u) +
+                              " * 64ull), (long long)lane" + d + ", w" + std::to_string(u) + d + ", NW)";
+        if (guard) s << "ok" << u << " ? " << e << " : 0ull;\n";
+        else s << e << ";\n";
+    };
+    // the words an external shift of node n reads, as (input, word offset)
+    auto shift_words = [&](int32_t n) {
+        std::vector<std::pair<int32_t, long long>> out;
+        const GNode& x = g.nodes[n];
+        const GNode& src = g.nodes[x.a];
+        if (src.op == NOp::Zero || src.op == NOp::Ones) return out;
+        const long long q = (long long)(x.k / 64);
+        const bool part = (x.k % 64) != 0;
+        if (x.op == NOp::Shl) {
+            out.push_back({x.a, -q});
+            if (part) out.push_back({x.a, -q - 1});
+        } else {
+            out.push_back({x.a, q});
+            if (part) out.push_back({x.a, q + 1});
+        }
+        return out;
+    };
+    auto emit_y = [&](int u, int32_t n, bool guard) {
+        const GNode& x = g.nodes[n];
+        const long long q = (long long)(x.k / 64);
+        const unsigned b = (unsigned)(x.k % 64);
+        const GNode& src = g.nodes[x.a];
+        auto word = [&](long long d) -> std::string {
+            if (src.op == NOp::Zero) return "0ull";
+            std::string wd = "w" + std::to_string(u) + (d < 0 ? " - " + std::to_string(-d) + "ll" : " + " + std::to_string(d) + "ll");
+            if (src.op == NOp::Ones) return "word_mask(" + wd + ", D.opw, D.opmask)";
+            if (d == 0) return "x" + std::to_string(x.a) + "_" + std::to_string(u);
+            return "z" + std::to_string(x.a) + "_" + (d < 0 ? "m" : "p") + std::to_string(std::llabs(d)) + "_" + std::to_string(u);
+        };
+        std::string e = x.op == NOp::Shl
+            ? "fsl(" + word(-q) + ", " + (b ? word(-q - 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)"
+            : "fsr(" + word(q) + ", " + (b ? word(q + 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)";
+        if (guard) s << "      const u64 y" << n << "_" << u << " = ok" << u << " ? opq(" << e << ") : 0ull;\n";
+        else s << "      const u64 y" << n << "_" << u << " = opq(" << e << ");\n";
+    };
+
+    // Row indices of U rows starting at row r (row u: `r + u`, or
+    // `r + u * istride` when interleaved; row u exists while its index is
+    // below `limit`), and, unless loads are lazy, every input word the rows
+    // need, issued together.
     auto emit_loads = [&](int UU, const std::string& limit) {
         for (int u = 0; u < UU; ++u) {
             // The row index goes through readfirstlane so that addresses are a
@@ -505,36 +565,10 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
             s << "      const long long w" << u << " = (long long)(ru" << u << " * 64ull + lane);\n";
             if (UU > 1) s << "      const bool ok" << u << " = " << rexpr << " < " << limit << ";\n";
             else s << "      const bool ok" << u << " = true;\n";
-            for (int32_t a : plain_inputs) {
-                const std::string rb = "urow(in" + std::to_string(input_index.at(a)) + " + ru" + std::to_string(u) + " * 64ull)";
-                if (UU > 1)
-                    s << "      const u64 x" << a << "_" << u << " = ok" << u << " ? lane_ld(" << rb << ", off8) : 0ull;\n";
-                else
-                    s << "      const u64 x" << a << "_" << u << " = lane_ld(" << rb << ", off8);\n";
-            }
-            for (auto& ow : off_words) {
-                const std::string d = ow.second < 0 ? " - " + std::to_string(-ow.second) + "ll" : " + " + std::to_string(ow.second) + "ll";
-                s << "      const u64 z" << ow.first << "_" << (ow.second < 0 ? "m" : "p") << std::llabs(ow.second) << "_" << u
-                  << " = ok" << u << " ? ldr(urow(in" << input_index.at(ow.first) << " + ru" << u << " * 64ull), (long long)lane"
-                  << d << ", w" << u << d << ", NW) : 0ull;\n";
-            }
-            for (int32_t n : ext_shifts) {
-                const GNode& x = g.nodes[n];
-                const long long q = (long long)(x.k / 64);
-                const unsigned b = (unsigned)(x.k % 64);
-                const GNode& src = g.nodes[x.a];
-                auto word = [&](long long d) -> std::string {
-                    if (src.op == NOp::Zero) return "0ull";
-                    std::string wd = "w" + std::to_string(u) + (d < 0 ? " - " + std::to_string(-d) + "ll" : " + " + std::to_string(d) + "ll");
-                    if (src.op == NOp::Ones) return "word_mask(" + wd + ", D.opw, D.opmask)";
-                    if (d == 0) return "x" + std::to_string(x.a) + "_" + std::to_string(u);
-                    return "z" + std::to_string(x.a) + "_" + (d < 0 ? "m" : "p") + std::to_string(std::llabs(d)) + "_" + std::to_string(u);
-                };
-                std::string e = x.op == NOp::Shl
-                    ? "fsl(" + word(-q) + ", " + (b ? word(-q - 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)"
-                    : "fsr(" + word(q) + ", " + (b ? word(q + 1) : std::string("0ull")) + ", " + std::to_string(b) + "u)";
-                s << "      const u64 y" << n << "_" << u << " = ok" << u << " ? opq(" << e << ") : 0ull;\n";
-            }
+            if (opt.lazy_loads) continue;
+            for (int32_t a : plain_inputs) emit_x(u, a, UU > 1);
+            for (auto& ow : off_words) emit_z(u, ow, true);
+            for (int32_t n : ext_shifts) emit_y(u, n, true);
         }
     };
 
@@ -557,6 +591,30 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
             return "x" + std::to_string(a) + "_" + std::to_string(u);
         };
         auto lref = [&](int32_t a) -> std::string { return in_kernel.count(a) ? "L" + std::to_string(a) : "0u"; };
+        // lazy loads: each input word right before its first use in the row
+        std::set<int32_t> have_x, have_y;
+        std::set<std::pair<int32_t, long long>> have_z;
+        auto need_x = [&](int32_t a) {
+            if (!opt.lazy_loads || !plain_inputs.count(a) || !have_x.insert(a).second) return;
+            emit_x(u, a, false);
+        };
+        auto need = [&](int32_t n) {
+            if (!opt.lazy_loads) return;
+            const GNode& x = g.nodes[n];
+            if ((x.op == NOp::Shl || x.op == NOp::Shr) && !in_kernel.count(x.a)) {
+                if (x.k == 0) { need_x(x.a); return; }
+                if (!have_y.insert(n).second) return;
+                for (auto& wd : shift_words(n)) {
+                    if (wd.second == 0) need_x(wd.first);
+                    else if (have_z.insert(wd).second) emit_z(u, wd, false);
+                }
+                emit_y(u, n, false);
+                return;
+            }
+            if (x.op == NOp::Input) { need_x(n); return; }
+            if (arity(x.op) >= 1 && !in_kernel.count(x.a)) need_x(x.a);
+            if (arity(x.op) >= 2 && !in_kernel.count(x.b)) need_x(x.b);
+        };
         auto lmax = [&](const std::string& x, const std::string& y) -> std::string {
             if (x == "0u") return y;
             if (y == "0u" || x == y) return x;
@@ -580,6 +638,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         };
         for (int32_t n : ks.nodes) {
             const GNode& x = g.nodes[n];
+            need(n);
             const std::string v = "v" + std::to_string(n), L = "L" + std::to_string(n);
             switch (x.op) {
                 case NOp::Zero: s << "        const u64 " << v << " = 0ull;\n"; break;
@@ -661,23 +720,31 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
             std::set<int32_t> rest;
             for (auto& kv : out_of) if (!done.count(kv.first)) rest.insert(kv.first);
             for (auto& kv : pop_of) if (!done.count(kv.first)) rest.insert(kv.first);
-            for (int32_t n : rest) sink(n);
+            for (int32_t n : rest) {
+                need(n);
+                sink(n);
+            }
         }
         s << "      }\n";
     };
 
-    // The state words as this chunk publishes them: lane 2i holds the low and
-    // lane 2i+1 the high half of state word i, each under the run's tag in
-    // the upper 32 bits, so one 64-bit store per lane publishes a word half
-    // with the run it belongs to and a reader needs no fence.
+    // The state words as this chunk publishes them: in group g (state words
+    // 32g .. 32g+31), lane 2j holds the low and lane 2j+1 the high half of
+    // state word 32g+j, each under the run's tag in the upper 32 bits, so one
+    // 64-bit store per lane publishes a word half with the run it belongs to
+    // and a reader needs no fence.
+    const int NG = (n_state + 31) / 32;
+    auto group_lanes = [&](int g) { return std::to_string(std::min(64, 2 * n_state - 64 * g)) + "u"; };
     auto emit_tagged = [&](const std::string& var) {
-        s << "    u32 " << var << "_v = 0u;\n";
+        for (int g = 0; g < NG; ++g) s << "    u32 " << var << "_v" << g << " = 0u;\n";
         for (auto& kv : state_of) {
-            const int i = kv.second;
-            s << "    " << var << "_v = lane == " << 2 * i << "u ? (u32)st" << i << " : " << var << "_v;\n";
-            s << "    " << var << "_v = lane == " << 2 * i + 1 << "u ? (u32)(st" << i << " >> 32) : " << var << "_v;\n";
+            const int i = kv.second, g = i / 32, j = i % 32;
+            const std::string vv = var + "_v" + std::to_string(g);
+            s << "    " << vv << " = lane == " << 2 * j << "u ? (u32)st" << i << " : " << vv << ";\n";
+            s << "    " << vv << " = lane == " << 2 * j + 1 << "u ? (u32)(st" << i << " >> 32) : " << vv << ";\n";
         }
-        s << "    const u64 " << var << " = ((u64)tag << 32) | " << var << "_v;\n";
+        for (int g = 0; g < NG; ++g)
+            s << "    const u64 " << var << g << " = ((u64)tag << 32) | " << var << "_v" << g << ";\n";
     };
     if (has_state) {
         // Warm-up (chunks after an instance's first): the rows before the
@@ -730,10 +797,18 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "    if (pass == 0u && !proven) {\n";
         s << "      if (lane == 0u) __hip_atomic_fetch_add(&kst[1], 1u, __ATOMIC_RELAXED, AGENT);\n";
         s << "      const u64* const prev = A.st_t + (u64)(c - 1u) * " << NS2 << ";\n";
-        s << "      u64 w = spec;\n";
+        for (int g = 0; g < NG; ++g) s << "      u64 w" << g << " = spec" << g << ";\n";
         s << "      for (u32 it = 0u;; ++it) {\n";
-        s << "        const u64 x = lane < " << NS2 << " ? aload64((u64*)prev + lane) : ((u64)tag << 32);\n";
-        s << "        if (ballot((u32)(x >> 32) != tag) == 0ull) { w = x; break; }\n";
+        s << "        bool stale = false;\n";
+        for (int g = 0; g < NG; ++g) {
+            s << "        const u64 x" << g << " = lane < " << group_lanes(g) << " ? aload64((u64*)prev + " << 64 * g
+              << "u + lane) : ((u64)tag << 32);\n";
+            s << "        stale |= ballot((u32)(x" << g << " >> 32) != tag) != 0ull;\n";
+        }
+        s << "        if (!stale) {\n";
+        for (int g = 0; g < NG; ++g) s << "          w" << g << " = x" << g << ";\n";
+        s << "          break;\n";
+        s << "        }\n";
         s << "        if (it >= A.spin_limit) {\n";
         s << "          if (lane == 0u) {\n";
         s << "            __hip_atomic_fetch_or(A.err, 1u, __ATOMIC_RELAXED, AGENT);\n";
@@ -743,12 +818,14 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "        }\n";
         s << "        __builtin_amdgcn_s_sleep(2);\n";
         s << "      }\n";
-        s << "      if (ballot(w != spec) != 0ull) {\n";
+        s << "      bool differs = false;\n";
+        for (int g = 0; g < NG; ++g) s << "      differs |= ballot(w" << g << " != spec" << g << ") != 0ull;\n";
+        s << "      if (differs) {\n";
         s << "        if (lane == 0u) __hip_atomic_fetch_add(&kst[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
         for (auto& kv : state_of) {
-            const int i = kv.second;
-            s << "        st" << i << " = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)w, " << 2 * i + 1
-              << ") << 32) | (u64)(u32)__builtin_amdgcn_readlane((int)(u32)w, " << 2 * i << ");\n";
+            const int i = kv.second, g = i / 32, j = i % 32;
+            s << "        st" << i << " = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)w" << g << ", " << 2 * j + 1
+              << ") << 32) | (u64)(u32)__builtin_amdgcn_readlane((int)(u32)w" << g << ", " << 2 * j << ");\n";
         }
         s << "        continue;\n";
         s << "      }\n";
@@ -757,7 +834,9 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         s << "    }\n";  // pass loop
         // the end state, final: this chunk's successor may be waiting for it
         emit_tagged("fin");
-        s << "    if (lane < " << NS2 << ") astore64(&A.st_t[(u64)c * " << NS2 << " + lane], fin);\n";
+        for (int g = 0; g < NG; ++g)
+            s << "    if (lane < " << group_lanes(g) << ") astore64(&A.st_t[(u64)c * " << NS2 << " + " << 64 * g
+              << "u + lane], fin" << g << ");\n";
     }
 
     // popcounts: one atomic per chunk into slot c mod 2^spread_log
@@ -806,7 +885,7 @@ std::string PlanOptions::key() const {
     std::ostringstream s;
     s << chunk_rows << "," << warmup_rows << "," << waves_per_block << "," << rows_per_step << "," << max_ops << ","
       << max_live << "," << occupancy << "," << words_per_lane << "," << store_order << "," << nontemporal << ","
-      << interleave << "," << persistent;
+      << interleave << "," << persistent << "," << agprs << "," << lazy_loads;
     return s.str();
 }
 
@@ -874,6 +953,11 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
     std::vector<std::vector<int32_t>> kernels;
     std::vector<int> peak_live;                    // per kernel: most in-kernel values live at once
     std::set<std::pair<int32_t, uint64_t>> loaded;  // (external operand, shift or 0) words of this kernel
+    // With lazy loads a loaded word takes a register only from its first use
+    // in the kernel to its operand's last use: the positions at which the
+    // words counted in `lload` are freed.
+    std::priority_queue<int32_t, std::vector<int32_t>, std::greater<int32_t>> lload_free;
+    int lload = 0;
     int live = 0, n_state = 0, k_pops = 0;
     std::vector<int> popped(N, 0);  // popcounts taken of each node
     for (int32_t p : g.pops) ++popped[p];
@@ -883,6 +967,10 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
         if (!needed[n]) continue;
         const GNode& x = g.nodes[n];
         if (is_free(x.op)) continue;
+        while (!lload_free.empty() && lload_free.top() < i) {
+            lload_free.pop();
+            --lload;
+        }
         const int K = (int)kernels.size() - 1;
         bool cut = kernels.empty();
         if (!cut) {
@@ -899,9 +987,10 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
             if (arity(x.op) >= 1 && !inside(x.a) && !is_const(x.a) && !loaded.count({x.a, sk})) ++new_loads;
             if (arity(x.op) >= 2 && !inside(x.b) && !is_const(x.b) && x.b != x.a && !loaded.count({x.b, 0})) ++new_loads;
             // a popcount keeps a 32-bit counter per lane: half a value
-            if (live + delta + (int)loaded.size() + new_loads + (k_pops + popped[n] + 1) / 2 > value_budget)
+            const int held = opt.lazy_loads ? lload : (int)loaded.size();
+            if (live + delta + held + new_loads + (k_pops + popped[n] + 1) / 2 > value_budget)
                 cut = true;
-            if (n_state >= kMaxStateWords && (x.op == NOp::Add || (stateful_shift(x) && inside(x.a)))) cut = true;
+            if (n_state >= max_state_words(opt) && (x.op == NOp::Add || (stateful_shift(x) && inside(x.a)))) cut = true;
         }
         if (cut) {
             kernels.emplace_back();
@@ -910,14 +999,21 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
             n_state = 0;
             k_pops = 0;
             loaded.clear();
+            lload = 0;
+            lload_free = {};
         }
         k_pops += popped[n];
         const int KK = (int)kernels.size() - 1;
         auto inside = [&](int32_t a) { return a >= 0 && kern_of[a] == KK; };
         {
             const uint64_t sk = (x.op == NOp::Shl || x.op == NOp::Shr) ? (x.op == NOp::Shr ? (1ull << 63) : 0) | x.k : 0;
-            if (arity(x.op) >= 1 && !inside(x.a) && !is_const(x.a)) loaded.insert({x.a, sk});
-            if (arity(x.op) >= 2 && !inside(x.b) && !is_const(x.b)) loaded.insert({x.b, 0});
+            auto load = [&](int32_t a, uint64_t shift) {
+                if (!loaded.insert({a, shift}).second || node_last_use[a] <= i) return;
+                ++lload;   // held until the operand's last use
+                lload_free.push(node_last_use[a]);
+            };
+            if (arity(x.op) >= 1 && !inside(x.a) && !is_const(x.a)) load(x.a, sk);
+            if (arity(x.op) >= 2 && !inside(x.b) && !is_const(x.b)) load(x.b, 0);
         }
         if (arity(x.op) >= 1 && inside(x.a) && node_last_use[x.a] == i) --live;
         if (arity(x.op) >= 2 && x.b != x.a && inside(x.b) && node_last_use[x.b] == i) --live;

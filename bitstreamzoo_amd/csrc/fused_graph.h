@@ -85,6 +85,8 @@ struct PlanOptions {
     int max_live = 96;
     int occupancy = 0;               // waves per SIMD the register budget allows; 0 = from an estimate
     bool persistent = false;         // grid of resident waves, each striding over the chunks
+    bool agprs = false;              // occupancy 1 only: values may also live in the 256 AGPRs
+    bool lazy_loads = false;         // load each input word at its first use in a row, not at the row's start
     std::string key() const;
 };
 

@@ -6,7 +6,7 @@ profiler to watch, and write the plan's per-kernel shapes next to it.
 
 The JSON holds every kernel's declared input and output stream counts (the
 bytes a kernel must move: 8 * words per stream), the chunk count and the
-chunks each kernel's fix-up launch redid, so per-dispatch counters (FETCH_SIZE,
+chunks each kernel ran again after its look-back, so per-dispatch counters (FETCH_SIZE,
 WRITE_SIZE) can be set against the planned traffic.  Kernels are compiled by
 worker processes first; the runs themselves are the only GPU work.
 """
@@ -51,14 +51,14 @@ def main(argv=None) -> int:
     ks = info["kernels"]
     summary = {"nodes": a.nodes, "dsl_ops": p.n_dsl_ops, "bitlength": case.bitlength, "schedule": opts,
                "n_chunks": info["n_chunks"], "kernels": ks,
-               "fixup_chunks": [plan.mismatches(k) for k in range(len(ks))]}
+               "rerun_chunks": [plan.mismatches(k) for k in range(len(ks))]}
     plan.close()
     res.free()
     ctx.close()
     if a.out:
         with open(a.out, "w") as f:
             json.dump(summary, f)
-    log(f"{len(ks)} kernels, {sum(summary['fixup_chunks'])} chunks redone")
+    log(f"{len(ks)} kernels, {sum(summary['rerun_chunks'])} chunks redone")
     return 0
 
 

@@ -232,10 +232,15 @@ def register_clean(info: dict) -> bool:
     (no AGPR use, no scratch); kernels that overflow spill values on every
     row, so the tuner does not use such schedules.  A kernel whose
     counts are missing or unknown (-1) counts as unclean.  (The library
-    itself refuses such plans; this is the tuner's own check of the info.)"""
+    itself refuses such plans; this is the tuner's own check of the info.)
+    A plan built with {"occupancy": 1, "agprs": 1} may keep values in AGPRs,
+    never in scratch."""
+    allow_agprs = info.get("agprs") == 1
     for k in info.get("kernels", []):
         vg, ag, sc = k.get("vgprs", -1), k.get("agprs", -1), k.get("scratch_bytes", -1)
-        if vg is None or ag is None or sc is None or vg <= 0 or ag != 0 or sc != 0:
+        if vg is None or ag is None or sc is None or vg <= 0 or ag < 0 or sc != 0:
+            return False
+        if ag != 0 and not allow_agprs:
             return False
     return True
 

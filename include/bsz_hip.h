@@ -182,12 +182,15 @@ int bsz_plan_run(bsz_plan* plan);
 /* Capture one run into a hipGraph; bsz_plan_replay launches it `times` times. */
 int bsz_plan_capture(bsz_plan* plan);
 int bsz_plan_replay(bsz_plan* plan, int times);
-/* Int output (a popcount) of instance inst after the last run (synchronous). */
+/* Int output (a popcount) of instance inst after the last run (synchronous).
+ * BSZ_E_RUNTIME when a chunk's look-back wait of that run reached its bound
+ * (option "spin_limit"): the run's outputs are then not exact. */
 int bsz_plan_read_int(bsz_plan* plan, int inst, const char* name, int64_t* out);
-/* Chunks whose speculative start state was wrong in the last run, kernel k. */
+/* Chunks of kernel k whose speculative start state was wrong in the last run
+ * (each ran again from its predecessor's end state, in the same launch). */
 int bsz_plan_mismatches(bsz_plan* plan, int kernel, int64_t* out);
 /* Chunks of kernel k whose warm-up rows did not prove their start state
- * exact in the last run (only these are checked by the fix-up launch). */
+ * exact in the last run (only these wait for their predecessor's end state). */
 int bsz_plan_unproven(bsz_plan* plan, int kernel, int64_t* out);
 /* BSZ_EXEC_FUSED (default) or BSZ_EXEC_PEROP: one kernel per DSL op over the
  * same recorded DAG and the same bound buffers (the plain comparator). */

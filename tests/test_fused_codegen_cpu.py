@@ -251,3 +251,28 @@ def test_plan_create_rejects_null_arguments():
     rc = L.bsz_plan_create(None, api.Program(w.source).h, api.Case.from_bscase(w.cases[0]).h, b'{"device_if":1}',
                            C.byref(h))
     assert rc != 0
+
+
+def test_lazy_loads_and_agprs_grow_partitions_without_spilling():
+    """Loading each input word at its first use lets a partition count it as
+    a value only up to its last use, and AGPRs (occupancy 1) give a wave 512
+    registers: a generated DAG then needs fewer, larger kernels, and the
+    compiled kernels keep every value in registers (AGPRs allowed, scratch
+    never)."""
+    from bitstreamzoo_amd import dag_gen, tuning
+    p = dag_gen.generate(3000, seed=21)
+    case = dag_gen.text_case(1 << 14, seed=22)
+    prog, shape = api.Program(p.source), tuning.shape_case([case])
+    base = {"warmup_rows": 4}
+    big = dict(base, lazy_loads=1, occupancy=1, agprs=1, max_live=160, rows_per_step=1)
+    d_base, d_lazy, d_big = api.describe_plans(prog, shape, [base, dict(base, lazy_loads=1), big])
+    traffic = [sum(k["inputs"] + k["outputs"] for k in d["kernels"]) for d in (d_base, d_lazy, d_big)]
+    assert len(d_big["kernels"]) < len(d_lazy["kernels"]) < len(d_base["kernels"])
+    assert traffic[2] < traffic[1] < traffic[0], traffic
+    assert max(k["state_words"] for k in d_big["kernels"]) > 32   # more state words with the AGPRs
+    info = api.compile_plan_only(prog, shape, big)
+    assert info["agprs"] == 1
+    assert all(k["scratch_bytes"] == 0 and k["vgprs"] > 0 for k in info["kernels"]), info["kernels"]
+    assert tuning.register_clean(info)
+    # a plan without the AGPR option never runs a kernel that uses them
+    assert not tuning.register_clean(dict(info, agprs=0)) or all(k["agprs"] == 0 for k in info["kernels"])

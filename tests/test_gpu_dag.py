@@ -13,7 +13,10 @@ from runner import run_model
 pytestmark = pytest.mark.gpu
 
 SCHEDULES = [{}, {"chunk_rows": 2, "rows_per_step": 3, "max_live": 24, "max_ops": 300},
-             {"chunk_rows": 8, "rows_per_step": 1, "max_live": 160, "occupancy": 1}]
+             {"chunk_rows": 8, "rows_per_step": 1, "max_live": 160, "occupancy": 1},
+             # input words loaded at their first use, and values kept in AGPRs too
+             {"chunk_rows": 4, "lazy_loads": 1},
+             {"chunk_rows": 8, "rows_per_step": 1, "max_live": 160, "occupancy": 1, "agprs": 1, "lazy_loads": 1}]
 
 
 def oracle_outputs(p, case):
