@@ -389,13 +389,21 @@ def bench_mixed(ctx, args):
     compile_s = time.perf_counter() - t1
     log(f"mixed: {n} programs, picks in {pick_s:.1f} s, compiled in {compile_s:.1f} s")
     residents, plans, defaults = [], [], []
-    bit_ops = alg = launches = 0
+    bit_ops = alg = launches = refused = 0
     try:
-        for (p, case), o in zip(built, picks):
+        for i, ((p, case), o) in enumerate(zip(built, picks)):
             r = tuning.Resident(ctx, [case])
             residents.append(r)
             prog = api.Program(p.source)
-            pl = api.Plan(ctx, prog, tuning.shape_case([case]), o)
+            try:
+                pl = api.Plan(ctx, prog, tuning.shape_case([case]), o)
+            except api.BszError as e:
+                if not tuning.refused(e):
+                    raise
+                # the model picked a schedule whose kernels would overflow
+                refused += 1
+                picks[i] = {}
+                pl = api.Plan(ctx, prog, tuning.shape_case([case]), {})
             r.bind(pl)
             plans.append(pl)
             d = api.Plan(ctx, prog, tuning.shape_case([case]), {})
@@ -436,7 +444,7 @@ def bench_mixed(ctx, args):
                        "pick_ms": round(pick_ms, 4), "autotuned_best_ms": round(float(min(m.ms)), 4),
                        "gap_pct": round(100 * (pick_ms / float(min(m.ms)) - 1), 2)})
         log(f"mixed: judged program {i}: pick {pick_ms:.3f} ms, autotuned best {min(m.ms):.3f} ms")
-    return {"programs": n, "nodes": [lo, hi], "log2_bytes": [blo, bhi], "ms_per_step": ms,
+    return {"programs": n, "nodes": [lo, hi], "log2_bytes": [blo, bhi], "ms_per_step": ms, "refused_picks": refused,
             "default_ms_per_step": default_ms, "launches_per_step": launches, "bit_ops": bit_ops,
             "alg_bytes": alg, "pick_s": round(pick_s, 2), "compile_s": round(compile_s, 1), "judged": judged}
 
@@ -688,7 +696,7 @@ def main():
             ex[name]["outputs_checked_against_perop"] = rr["outputs_checked"]
         if name == "mixed":
             for k in ("programs", "nodes", "log2_bytes", "default_ms_per_step", "launches_per_step", "pick_s",
-                      "compile_s", "judged"):
+                      "compile_s", "refused_picks", "judged"):
                 ex[name][k] = rr[k]
         if "floor_ms" in rr:
             # the same bytes in the cheapest single launch (r = M & C)

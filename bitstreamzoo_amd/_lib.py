@@ -133,6 +133,10 @@ class BszError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[{code}] {msg}")
         self.code = code
+        self.msg = msg
+
+    def __reduce__(self):   # picklable, so worker processes can return it
+        return (BszError, (self.code, self.msg))
 
 
 def check(rc: int, ctx=None):

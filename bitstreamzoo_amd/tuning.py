@@ -47,8 +47,33 @@ def _compile_worker(job):
     import os
     os.environ["BSZ_COMPILE_THREADS"] = "1"
     source, desc, options = job
-    api.compile_plan_only(api.Program(source), _shape_from_desc(desc), options)
+    try:
+        api.compile_plan_only(api.Program(source), _shape_from_desc(desc), options)
+    except api.BszError as e:
+        # a schedule the library refuses is an answer, not a failure of the
+        # pool: the plan built from it later reports the same refusal
+        if "schedule refused" in str(e):
+            return str(e)
+        raise RuntimeError(str(e)) from None
     return True
+
+
+def _drain(futs, log, what: str, t0: float) -> None:
+    """Waits for every job; on the first failure the pool is shut down and
+    dropped (a broken pool left behind would hold the process at exit)."""
+    import concurrent.futures as cf
+    global _POOL
+    every = max(1, len(futs) // 10)
+    try:
+        for i, f in enumerate(cf.as_completed(futs)):
+            f.result()
+            if log and ((i + 1) % every == 0 or i + 1 == len(futs)):
+                log(f"compiled {i + 1}/{len(futs)} {what} ({time.perf_counter() - t0:.0f} s)")
+    except BaseException:
+        if _POOL is not None:
+            _POOL[1].shutdown(wait=False, cancel_futures=True)
+            _POOL = None
+        raise
 
 
 def precompile(source: str, cases: list[BsCase], options_list: list[dict], processes: int = 16,
@@ -69,11 +94,7 @@ def precompile(source: str, cases: list[BsCase], options_list: list[dict], proce
             o.update(compile_shard=r, compile_shards=shards)
             jobs.append((source, desc, o))
     futs = [_pool(processes).submit(_compile_worker, j) for j in jobs]
-    every = max(1, len(jobs) // 10)
-    for i, f in enumerate(cf.as_completed(futs)):
-        f.result()
-        if log and ((i + 1) % every == 0 or i + 1 == len(jobs)):
-            log(f"compiled {i + 1}/{len(jobs)} shards ({time.perf_counter() - t0:.0f} s)")
+    _drain(futs, log, "shards", t0)
     return time.perf_counter() - t0
 
 
@@ -85,11 +106,7 @@ def precompile_many(jobs: list, processes: int = 16, log=None) -> float:
     t0 = time.perf_counter()
     futs = [_pool(processes).submit(_compile_worker, (src, _shape_desc(cases), dict(opts)))
             for src, cases, opts in jobs]
-    every = max(1, len(futs) // 10)
-    for i, f in enumerate(cf.as_completed(futs)):
-        f.result()
-        if log and ((i + 1) % every == 0 or i + 1 == len(futs)):
-            log(f"compiled {i + 1}/{len(futs)} programs ({time.perf_counter() - t0:.0f} s)")
+    _drain(futs, log, "programs", t0)
     return time.perf_counter() - t0
 
 

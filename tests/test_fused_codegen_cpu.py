@@ -276,3 +276,18 @@ def test_lazy_loads_and_agprs_grow_partitions_without_spilling():
     assert tuning.register_clean(info)
     # a plan without the AGPR option never runs a kernel that uses them
     assert not tuning.register_clean(dict(info, agprs=0)) or all(k["agprs"] == 0 for k in info["kernels"])
+
+
+def test_refused_schedule_is_an_answer_of_the_compile_pool():
+    """Compiling a schedule the library refuses in the worker pool returns
+    the refusal instead of breaking the pool, and the pool keeps working."""
+    from bitstreamzoo_amd import dag_gen, tuning
+    p = dag_gen.generate(1200, seed=2)
+    case = dag_gen.text_case(5000, seed=2)
+    refused = {"chunk_rows": 8, "rows_per_step": 1, "max_live": 160, "occupancy": 1}
+    tuning.precompile(p.source, [case], [refused, {}], processes=2)
+    msg = tuning._compile_worker((p.source, tuning._shape_desc([case]), refused))
+    assert isinstance(msg, str) and "schedule refused" in msg
+    import pickle
+    e = pickle.loads(pickle.dumps(BszError(-3, "schedule refused: x")))
+    assert e.code == -3 and "schedule refused" in str(e)
