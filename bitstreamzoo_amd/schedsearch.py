@@ -310,6 +310,50 @@ def mixed_batch(ctx: api.Context, model: costmodel.CostModel, specs: list[Progra
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
 
+def evaluate_large(ctx: api.Context, model: costmodel.CostModel, nodes: int, log2_bytes: int,
+                   candidates: list[dict], judge: int, processes: int = 16, seed: int = 0, log=None) -> dict:
+    """The model on a program far larger than anything it was trained on
+    (the config-4 generator DAG): its pick is the best-predicted candidate
+    that compiles within its register file, found from plan features alone;
+    the measured autotuner then times a subset of `judge` candidates (the
+    pick, the benchmark's default schedule and random others) as judge."""
+    t0 = time.perf_counter()
+    p = dag_gen.generate(nodes, seed=4)
+    case = dag_gen.text_case(1 << log2_bytes, seed=11)
+    gen_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    prog = api.Program(p.source)
+    F = costmodel.features_batch(prog, tuning.shape_case([case]), candidates, max(1, (case.bitlength + 63) // 64), 1)
+    pred = model.predict_ms(F)
+    predict_s = time.perf_counter() - t1
+    if log:
+        log(f"large: {nodes} nodes, features and predictions for {len(candidates)} candidates in {predict_s:.1f} s")
+    t2 = time.perf_counter()
+    pick = model_pick(ctx, p.source, case, candidates, pred, processes, top=1)
+    model_s = predict_s + (time.perf_counter() - t2)
+    pick_opts, pick_ms = pick.options[0], float(pick.ms[0])
+    if log:
+        log(f"large: pick {pick_opts} {pick_ms:.1f} ms ({model_s:.1f} s with its compile and timing)")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    default = {"warmup_rows": 4}
+    others = [candidates[i] for i in rng.permutation(len(candidates)).tolist()
+              if candidates[i] not in (pick_opts, default)]
+    trial = [pick_opts, default] + others[:max(0, judge - 2)]
+    t3 = time.perf_counter()
+    m = measure(ctx, p.source, case, trial, processes=processes, log=log, allow_empty=True)
+    tune_s = time.perf_counter() - t3
+    by = {json.dumps(o, sort_keys=True): ms for o, ms in zip(m.options, m.ms)}
+    best = min(m.ms) if m.ms else pick_ms
+    default_ms = by.get(json.dumps(default, sort_keys=True))
+    return {"nodes": nodes, "dsl_ops": p.n_dsl_ops, "bytes": 1 << log2_bytes, "candidates": len(candidates),
+            "pick": pick_opts, "pick_ms": round(pick_ms, 2),
+            "judged": [{"options": o, "ms": round(float(ms), 2)} for o, ms in zip(m.options, m.ms)],
+            "best_judged_ms": round(float(best), 2), "gap_pct": round(100 * (pick_ms / best - 1), 2),
+            "default_ms": round(float(default_ms), 2) if default_ms is not None else None,
+            "model_s": round(model_s, 1), "judge_s": round(tune_s, 1), "generate_s": round(gen_s, 1),
+            "predicted_pick_ms": round(float(np.min(pred)), 2)}
+
+
 def load_training_data():
     """Every committed autotuner measurement file (data/schedule_runtimes*.npz)."""
     import glob
@@ -347,6 +391,10 @@ def main(argv=None) -> int:
     ap.add_argument("--max-log2-bytes", type=float, default=24)
     ap.add_argument("--collect-only", action="store_true")
     ap.add_argument("--mixed", type=int, default=0, help="run a mixed batch of N programs instead of --eval")
+    ap.add_argument("--large", type=int, default=0,
+                    help="evaluate on one generated DAG of N nodes (the config-4 generator) instead of --eval")
+    ap.add_argument("--large-log2-bytes", type=int, default=27)
+    ap.add_argument("--judge", type=int, default=4, help="candidates the autotuner times as judge with --large")
     ap.add_argument("--pick-top", type=int, default=PICK_TOP,
                     help="best-predicted candidates the model's pick compiles and times")
     a = ap.parse_args(argv)
@@ -374,6 +422,13 @@ def main(argv=None) -> int:
     loss = model.fit(X, y)
     log(f"trained on {len(y)} rows in {model.train_seconds:.2f} s, loss {loss:.4f}")
     cands = sample_space(a.candidates, 12345)
+    if a.large:
+        r = evaluate_large(ctx, model, a.large, a.large_log2_bytes, cands, a.judge, processes=a.processes,
+                           seed=a.seed, log=log)
+        r.update({"train_rows": int(len(y)), "train_s": round(model.train_seconds, 2)})
+        print(json.dumps(r), flush=True)
+        ctx.close()
+        return 0
     if a.mixed:
         specs = random_specs(a.mixed, a.seed + 200, nodes=(500, 5000), log2_bytes=(16, 22))
         r = mixed_batch(ctx, model, specs, cands, processes=a.processes, seed=a.seed, log=log, pick_top=a.pick_top)
