@@ -28,9 +28,15 @@ public:
 
     RunResult run() {
         walk();
+        // The variable count is taken now; every stream that is not a program
+        // output is then released, so the last flush materialises the outputs
+        // alone (the same graph a recorded plan of the program builds) rather
+        // than every variable still in scope.
+        const int64_t nv = count_variables();
+        release_non_outputs();
         E.flush();
         settle_predicated();
-        return collect();
+        return collect(nv);
     }
 
     RecordedOutputs record() {
@@ -73,7 +79,24 @@ private:
         for (const Stmt& s : P.stmts) exec(s);
     }
 
-    RunResult collect() {
+    int64_t count_variables() const {
+        int64_t nv = 0;
+        for (auto& v : vars) nv += v ? 1 : 0;
+        for (auto& a : arrays) nv += (int64_t)a.second.size();
+        return nv;
+    }
+
+    void release_non_outputs() {
+        std::vector<char> keep(vars.size(), 0);
+        for (Sym o : P.outputs) keep[o] = 1;
+        for (size_t s = 0; s < vars.size(); ++s)
+            if (!keep[s]) vars[s].reset();
+        for (auto& a : arrays)
+            if (!keep[a.first])
+                for (auto& e : a.second) e.second.reset();
+    }
+
+    RunResult collect(int64_t nv) {
         RunResult r;
         for (Sym o : P.outputs) {
             const std::string& nm = P.syms.names[o];
@@ -103,9 +126,6 @@ private:
         r.op_count = ops;
         for (int i = 0; i < OP_N; ++i)
             if (mix[i]) r.op_mix[kOpNames[i]] = mix[i];
-        int64_t nv = 0;
-        for (auto& v : vars) nv += v ? 1 : 0;
-        for (auto& a : arrays) nv += (int64_t)a.second.size();
         r.n_variables = nv - (int64_t)absent_vars.size() - (int64_t)absent_elems.size();
         return r;
     }

@@ -12,7 +12,7 @@ import pytest
 
 import progfuzz
 import regex_ifs
-from bitstreamzoo_amd import api
+from bitstreamzoo_amd import api, tuning
 from cpu_models import Oracle
 from hip_models import assert_same, ctx
 from runner import run_model
@@ -45,8 +45,19 @@ def _hip(src, case, names, c=None):
     return out, res.op_count, res.json()
 
 
+@pytest.fixture(scope="module")
+def regex_program_compiled():
+    """The kernels of the 200- and 60-pattern programs, compiled up front by
+    16 worker processes (the plan bsz_run builds, recorded with its `if`
+    bodies as device selections), so no test spends minutes in hipRTC."""
+    for n_patterns, seed in ((200, 7), (60, 9)):
+        src = regex_ifs.program(n_patterns, seed=seed)
+        case = regex_ifs.text_case(5000, seed=3, alphabet=3)
+        tuning.precompile(src, [case], [{"device_if": 1}], processes=16)
+
+
 @pytest.mark.parametrize("n_bytes,alphabet", [(20000, 6), (20000, 3), (4096 * 64 + 7, 4)])
-def test_regex_shaped_program_runs_fused_without_host_decisions(n_bytes, alphabet):
+def test_regex_shaped_program_runs_fused_without_host_decisions(regex_program_compiled, n_bytes, alphabet):
     """The same 200-pattern program (its kernels compile once) over texts
     where most guards hold, few hold, and over 4096 rows."""
     src = regex_ifs.program(200, seed=7)
@@ -62,7 +73,7 @@ def test_regex_shaped_program_runs_fused_without_host_decisions(n_bytes, alphabe
     assert n_if >= 300
 
 
-def test_variable_count_follows_the_taken_branches():
+def test_variable_count_follows_the_taken_branches(regex_program_compiled):
     """Streams first declared inside a body exist only where the reference ran
     the body: the reported variable count equals the per-op executor's,
     which decides every `if` on the host."""
