@@ -335,6 +335,10 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     }
     ks.words_per_lane = wide ? 2 : 1;
     ks.rows_per_step = wide ? 2 * U : U;
+    // A stateful kernel whose step covers the whole chunk and whose loads
+    // for the warm-up rows and the chunk fit together issues them in one
+    // step, so the chunk's loads do not wait for the warm-up's.
+    const bool joined = has_state && !opt.interleave && U == CR && WR + CR <= u_cap && !opt.lazy_loads;
     ks.chunk_rows = CR;
 
     std::ostringstream s;
@@ -746,7 +750,95 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
         for (int g = 0; g < NG; ++g)
             s << "    const u64 " << var << g << " = ((u64)tag << 32) | " << var << "_v" << g << ";\n";
     };
-    if (has_state) {
+    // Look-back (an unproven chunk, after its rows ran from the speculative
+    // state): wait for the predecessor's end state of this run; when it
+    // differs from the assumed one, load it into the state words and leave
+    // the emitted code inside `if (differs) {`, where the caller runs the
+    // chunk's rows again.
+    auto emit_lookback = [&]() {
+        s << "      if (lane == 0u) __hip_atomic_fetch_add(&kst[1], 1u, __ATOMIC_RELAXED, AGENT);\n";
+        s << "      const u64* const prev = A.st_t + (u64)(c - 1u) * " << NS2 << ";\n";
+        for (int g = 0; g < NG; ++g) s << "      u64 w" << g << " = spec" << g << ";\n";
+        s << "      for (u32 it = 0u;; ++it) {\n";
+        s << "        bool stale = false;\n";
+        for (int g = 0; g < NG; ++g) {
+            s << "        const u64 x" << g << " = lane < " << group_lanes(g) << " ? aload64((u64*)prev + " << 64 * g
+              << "u + lane) : ((u64)tag << 32);\n";
+            s << "        stale |= ballot((u32)(x" << g << " >> 32) != tag) != 0ull;\n";
+        }
+        s << "        if (!stale) {\n";
+        for (int g = 0; g < NG; ++g) s << "          w" << g << " = x" << g << ";\n";
+        s << "          break;\n";
+        s << "        }\n";
+        s << "        if (it >= A.spin_limit) {\n";
+        s << "          if (lane == 0u) {\n";
+        s << "            __hip_atomic_fetch_or(A.err, 1u, __ATOMIC_RELAXED, AGENT);\n";
+        s << "            __hip_atomic_fetch_add(&kst[2], 1u, __ATOMIC_RELAXED, AGENT);\n";
+        s << "          }\n";
+        s << "          break;\n";
+        s << "        }\n";
+        s << "        __builtin_amdgcn_s_sleep(2);\n";
+        s << "      }\n";
+        s << "      bool differs = false;\n";
+        for (int g = 0; g < NG; ++g) s << "      differs |= ballot(w" << g << " != spec" << g << ") != 0ull;\n";
+        s << "      if (differs) {\n";
+        s << "        if (lane == 0u) __hip_atomic_fetch_add(&kst[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
+        for (auto& kv : state_of) {
+            const int i = kv.second, g = i / 32, j = i % 32;
+            s << "        st" << i << " = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)w" << g << ", " << 2 * j + 1
+              << ") << 32) | (u64)(u32)__builtin_amdgcn_readlane((int)(u32)w" << g << ", " << 2 * j << ");\n";
+        }
+    };
+    // the end state, final: this chunk's successor may be waiting for it
+    auto emit_publish = [&]() {
+        emit_tagged("fin");
+        for (int g = 0; g < NG; ++g)
+            s << "    if (lane < " << group_lanes(g) << ") astore64(&A.st_t[(u64)c * " << NS2 << " + " << 64 * g
+              << "u + lane], fin" << g << ");\n";
+    };
+    if (joined) {
+        // One step: rows r_begin - WR .. r_begin + CR - 1, loads issued
+        // together.  Warm-up rows before the instance's first row (or any,
+        // for its first chunk) do not exist; a warm-up that starts at the
+        // instance's first row reaches the true state.
+        s << "    u32 proven = 1u;\n";
+        if (track)
+            for (auto& kv : state_of) s << "    u32 xs" << kv.second << " = 0u;\n";
+        for (int j = 0; j < WR + CR; ++j) {
+            const std::string row = "(long long)r_begin - " + std::to_string(WR - j) + "ll";
+            if (j < WR)
+                s << "      const bool ok" << j << " = cc > 0u && " << row << " >= 0ll;\n";
+            else
+                s << "      const bool ok" << j << " = r_begin + " << (j - WR) << "ull < r_end;\n";
+            s << "      const u64 ru" << j << " = ok" << j << " ? (u64)uni((u32)(" << row << ")) : 0ull;\n";
+            s << "      const long long w" << j << " = (long long)(ru" << j << " * 64ull + lane);\n";
+            for (int32_t a : plain_inputs) emit_x(j, a, true);
+            for (auto& ow : off_words) emit_z(j, ow, true);
+            for (int32_t n : ext_shifts) emit_y(j, n, true);
+        }
+        for (int j = 0; j < WR; ++j) emit_compute(j, true);
+        s << "    proven = (cc == 0u || r_begin <= " << WR << "ull) ? 1u : " << (track ? "1u" : "0u") << ";\n";
+        if (track) {
+            s << "    if (cc > 0u && r_begin > " << WR << "ull) {\n";
+            for (auto& kv : state_of) s << "      proven &= xs" << kv.second << ";\n";
+            s << "    }\n";
+        }
+        emit_tagged("spec");
+        for (int p = 0; p < NPOP; ++p) s << "    u32 pc" << p << " = 0u;\n";
+        for (int j = WR; j < WR + CR; ++j) emit_compute(j, false);
+        // an unproven chunk looks back; it runs its rows again from its
+        // predecessor's end state when that differs from the one assumed
+        s << "    if (!proven) {\n";
+        emit_lookback();
+        for (int p = 0; p < NPOP; ++p) s << "        pc" << p << " = 0u;\n";
+        s << "    for (u64 r = r_begin; r < r_end; r += " << U << "ull) {\n";
+        emit_loads(U, "r_end");
+        for (int u = 0; u < U; ++u) emit_compute(u, false);
+        s << "    }\n";  // row loop
+        s << "      }\n";  // differs
+        s << "    }\n";  // !proven
+        emit_publish();
+    } else if (has_state) {
         // Warm-up (chunks after an instance's first): the rows before the
         // chunk, from a zero state, storing nothing.  A warm-up that starts at
         // the instance's first row reaches the true state; otherwise the
@@ -783,6 +875,7 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     } else {
         for (int p = 0; p < NPOP; ++p) s << "    u32 pc" << p << " = 0u;\n";
     }
+    if (!joined) {
     if (inter) {
         s << "    const u64 istride = (u64)D.n_chunks;\n";
         s << "    for (u64 r = cc; r < D.rows; r += istride * " << U << "ull) {\n";
@@ -795,48 +888,14 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     s << "    }\n";  // row loop
     if (has_state) {
         s << "    if (pass == 0u && !proven) {\n";
-        s << "      if (lane == 0u) __hip_atomic_fetch_add(&kst[1], 1u, __ATOMIC_RELAXED, AGENT);\n";
-        s << "      const u64* const prev = A.st_t + (u64)(c - 1u) * " << NS2 << ";\n";
-        for (int g = 0; g < NG; ++g) s << "      u64 w" << g << " = spec" << g << ";\n";
-        s << "      for (u32 it = 0u;; ++it) {\n";
-        s << "        bool stale = false;\n";
-        for (int g = 0; g < NG; ++g) {
-            s << "        const u64 x" << g << " = lane < " << group_lanes(g) << " ? aload64((u64*)prev + " << 64 * g
-              << "u + lane) : ((u64)tag << 32);\n";
-            s << "        stale |= ballot((u32)(x" << g << " >> 32) != tag) != 0ull;\n";
-        }
-        s << "        if (!stale) {\n";
-        for (int g = 0; g < NG; ++g) s << "          w" << g << " = x" << g << ";\n";
-        s << "          break;\n";
-        s << "        }\n";
-        s << "        if (it >= A.spin_limit) {\n";
-        s << "          if (lane == 0u) {\n";
-        s << "            __hip_atomic_fetch_or(A.err, 1u, __ATOMIC_RELAXED, AGENT);\n";
-        s << "            __hip_atomic_fetch_add(&kst[2], 1u, __ATOMIC_RELAXED, AGENT);\n";
-        s << "          }\n";
-        s << "          break;\n";
-        s << "        }\n";
-        s << "        __builtin_amdgcn_s_sleep(2);\n";
-        s << "      }\n";
-        s << "      bool differs = false;\n";
-        for (int g = 0; g < NG; ++g) s << "      differs |= ballot(w" << g << " != spec" << g << ") != 0ull;\n";
-        s << "      if (differs) {\n";
-        s << "        if (lane == 0u) __hip_atomic_fetch_add(&kst[0], 1u, __ATOMIC_RELAXED, AGENT);\n";
-        for (auto& kv : state_of) {
-            const int i = kv.second, g = i / 32, j = i % 32;
-            s << "        st" << i << " = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)w" << g << ", " << 2 * j + 1
-              << ") << 32) | (u64)(u32)__builtin_amdgcn_readlane((int)(u32)w" << g << ", " << 2 * j << ");\n";
-        }
+        emit_lookback();
         s << "        continue;\n";
         s << "      }\n";
         s << "    }\n";
         s << "    break;\n";
         s << "    }\n";  // pass loop
-        // the end state, final: this chunk's successor may be waiting for it
-        emit_tagged("fin");
-        for (int g = 0; g < NG; ++g)
-            s << "    if (lane < " << group_lanes(g) << ") astore64(&A.st_t[(u64)c * " << NS2 << " + " << 64 * g
-              << "u + lane], fin" << g << ");\n";
+        emit_publish();
+    }
     }
 
     // popcounts: one atomic per chunk into slot c mod 2^spread_log
