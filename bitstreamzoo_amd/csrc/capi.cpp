@@ -230,7 +230,6 @@ int bsz_op_popcount(bsz_ctx* ctx, const uint64_t* src, int64_t bl, int64_t* out)
     return guard(ctx, [&] {
         Geom g = geom_of(bl);
         ensure_scalar_state(ctx, g);
-        launch_zero_u64(ctx->s, ctx->slot, kPopSlotWords);
         launch_popcount(ctx->s, ctx->slot, src, g);
         *out = (int64_t)read_popcount_slots(ctx->s, ctx->slot);
     });

@@ -132,14 +132,13 @@ constexpr uint64_t kNoAllocMaxBits = 1ull << 34;
 void setup_noalloc_scratch(int device);
 void launch_add_noalloc(hipStream_t s, int device, uint64_t* dst, const uint64_t* a, const uint64_t* b,
                         const Geom& g);
-// A popcount accumulator: kPopSlots u64 slots kPopSlotStride words apart
-// (kPopSlotWords words in all, zeroed by the caller before the launch).
-constexpr int kPopSlots = 64;
-constexpr int kPopSlotStride = 16;
-constexpr size_t kPopSlotWords = (size_t)kPopSlots * kPopSlotStride;
-// popcount bounded to the bitlength, added into the accumulator `slots`.
+// A popcount's partial sums: one u64 per block of its fixed grid of
+// kPopSlotWords blocks, each written (not added) by its block, so the buffer
+// needs no clearing and takes no atomics.
+constexpr size_t kPopSlotWords = 1024;
+// popcount bounded to the bitlength, as partial sums into `slots`.
 void launch_popcount(hipStream_t s, unsigned long long* slots, const uint64_t* src, const Geom& g);
-// The accumulator's total (synchronous).
+// The sum of the partials (synchronous).
 uint64_t read_popcount_slots(hipStream_t s, const unsigned long long* slots);
 // is_nonzero bounded to the bitlength, or-ed into *slot (zeroed by the caller).
 void launch_nonzero(hipStream_t s, unsigned long long* slot, const uint64_t* src, const Geom& g);

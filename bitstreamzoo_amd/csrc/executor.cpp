@@ -609,7 +609,6 @@ public:
         auto p = std::make_shared<PerOpPop>();
         p->dev = next_slots(kPopSlotWords);
         p->s = s_;
-        launch_zero_u64(s_, p->dev, kPopSlotWords);
         launch_popcount(s_, p->dev, a->buf->ptr, g_);
         return p;
     }

@@ -81,6 +81,7 @@ FusedOptions FusedOptions::from_json_obj(const Json& j) {
     get("persistent", o.persistent);
     get("agprs", o.agprs);
     get("lazy_loads", o.lazy_loads);
+    get("join_warmup", o.join_warmup);
     get("compile", o.compile);
     get("compile_shard", o.compile_shard);
     get("compile_shards", o.compile_shards);
@@ -106,6 +107,7 @@ PlanOptions to_plan_options(const FusedOptions& f) {
     p.persistent = f.persistent != 0;
     p.agprs = f.agprs != 0 && p.occupancy == 1;
     p.lazy_loads = f.lazy_loads != 0;
+    p.join_warmup = f.join_warmup != 0;
     return p;
 }
 
@@ -1478,7 +1480,6 @@ void FusedPlan::run() {
             }
         }
         if (!g.pops.empty()) {
-            launch_zero_u64(I.s, P.pops, g.pops.size() * kPopSlotWords);
             for (size_t p = 0; p < g.pops.size(); ++p)
                 launch_popcount(I.s, P.pops + p * kPopSlotWords, ptr_of(g.pops[p]), geo);
         }

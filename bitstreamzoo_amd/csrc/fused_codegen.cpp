@@ -335,10 +335,11 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     }
     ks.words_per_lane = wide ? 2 : 1;
     ks.rows_per_step = wide ? 2 * U : U;
-    // A stateful kernel whose step covers the whole chunk and whose loads
-    // for the warm-up rows and the chunk fit together issues them in one
-    // step, so the chunk's loads do not wait for the warm-up's.
-    const bool joined = has_state && !opt.interleave && U == CR && WR + CR <= u_cap && !opt.lazy_loads;
+    // On request, a stateful kernel whose step covers the whole chunk and
+    // whose loads for the warm-up rows and the chunk fit together issues
+    // them in one step, so the chunk's loads do not wait for the warm-up's.
+    const bool joined = opt.join_warmup && has_state && !opt.interleave && U == CR && WR + CR <= u_cap &&
+                        !opt.lazy_loads;
     ks.chunk_rows = CR;
 
     std::ostringstream s;
@@ -944,7 +945,7 @@ std::string PlanOptions::key() const {
     std::ostringstream s;
     s << chunk_rows << "," << warmup_rows << "," << waves_per_block << "," << rows_per_step << "," << max_ops << ","
       << max_live << "," << occupancy << "," << words_per_lane << "," << store_order << "," << nontemporal << ","
-      << interleave << "," << persistent << "," << agprs << "," << lazy_loads;
+      << interleave << "," << persistent << "," << agprs << "," << lazy_loads << "," << join_warmup;
     return s.str();
 }
 

@@ -87,6 +87,7 @@ struct PlanOptions {
     bool persistent = false;         // grid of resident waves, each striding over the chunks
     bool agprs = false;              // occupancy 1 only: values may also live in the 256 AGPRs
     bool lazy_loads = false;         // load each input word at its first use in a row, not at the row's start
+    bool join_warmup = false;        // a stateful kernel whose step spans its chunk loads warm-up and chunk together
     std::string key() const;
 };
 

@@ -239,9 +239,18 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
     return v;
 }
 
-// Bounded popcount: 16 bytes per lane per step, v_bcnt, a wave reduction,
-// then one atomic add per block into slot (block mod kPopSlots) of the
-// accumulator, slots 128 bytes apart, so no address takes more than a few.
+// Bounded popcount over a fixed grid of kPopSlotWords blocks: 16 bytes per
+// lane per load, two loads in flight per lane, v_bcnt, a wave and a block
+// reduction, and one plain store of the block's sum into slots[block].
+// Every block stores (0 when it had no words), so the partials need no
+// clearing and no address takes an atomic.
+__device__ __forceinline__ uint64_t pop_pair(const ulonglong2 v, uint64_t i, uint64_t lw, uint64_t lastmask) {
+    const uint64_t w = 2 * i;
+    uint64_t c = (uint64_t)__popcll(w + 1 == lw ? (v.x & lastmask) : v.x);
+    if (w + 1 < lw) c += (uint64_t)__popcll(w + 2 == lw ? (v.y & lastmask) : v.y);
+    return c;
+}
+
 __global__ __launch_bounds__(256) void k_popcount(unsigned long long* __restrict__ slots,
                                                   const uint64_t* __restrict__ src, uint64_t lw,
                                                   uint64_t lastmask) {
@@ -249,19 +258,17 @@ __global__ __launch_bounds__(256) void k_popcount(unsigned long long* __restrict
     uint64_t acc = 0;
     const uint64_t pairs = (lw + 1) / 2;   // the buffer is row-padded: word lw (if odd) exists and is 0
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < pairs; i += stride) {
-        const ulonglong2 v = reinterpret_cast<const ulonglong2*>(src)[i];
-        const uint64_t w = 2 * i;
-        acc += (uint64_t)__popcll(w + 1 == lw ? (v.x & lastmask) : v.x);
-        if (w + 1 < lw) acc += (uint64_t)__popcll(w + 2 == lw ? (v.y & lastmask) : v.y);
+    const ulonglong2* p = reinterpret_cast<const ulonglong2*>(src);
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + stride < pairs; i += 2 * stride) {
+        const ulonglong2 v0 = p[i], v1 = p[i + stride];
+        acc += pop_pair(v0, i, lw, lastmask) + pop_pair(v1, i + stride, lw, lastmask);
     }
+    if (i < pairs) acc += pop_pair(p[i], i, lw, lastmask);
     acc = wave_sum(acc);
     if (lane_id() == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint64_t t = part[0] + part[1] + part[2] + part[3];
-        if (t) atomicAdd(&slots[(blockIdx.x % kPopSlots) * kPopSlotStride], (unsigned long long)t);
-    }
+    if (threadIdx.x == 0) slots[blockIdx.x] = (unsigned long long)(part[0] + part[1] + part[2] + part[3]);
 }
 
 __global__ __launch_bounds__(256) void k_nonzero(unsigned long long* __restrict__ slot,
@@ -407,19 +414,19 @@ void launch_add_noalloc(hipStream_t s, int device, uint64_t* dst, const uint64_t
 }
 
 void launch_popcount(hipStream_t s, unsigned long long* slots, const uint64_t* src, const Geom& g) {
-    if (g.bitlength <= 0) return;  // bsim: popcount of an unbounded case is 0 (cuda_backend.cuh:88)
-    // about 8 words per lane: enough bytes in flight, few atomics
-    const unsigned grid = std::max(1u, std::min<unsigned>(4096u, blocks_for((g.lw + 1) / 2, 256 * 4)));
-    k_popcount<<<grid, 256, 0, s>>>(slots, src, g.lw, g.lastmask);
+    // bsim: popcount of an unbounded case is 0 (cuda_backend.cuh:88); the
+    // grid still runs, over no words, so that every partial is written
+    const uint64_t lw = g.bitlength > 0 ? g.lw : 0;
+    k_popcount<<<(unsigned)kPopSlotWords, 256, 0, s>>>(slots, src, lw, g.lastmask);
     check_launch();
 }
 
 uint64_t read_popcount_slots(hipStream_t s, const unsigned long long* slots) {
-    std::vector<unsigned long long> h((size_t)kPopSlots * kPopSlotStride);
+    std::vector<unsigned long long> h(kPopSlotWords);
     BSZ_HIP(hipMemcpyAsync(h.data(), slots, h.size() * 8, hipMemcpyDeviceToHost, s));
     BSZ_HIP(hipStreamSynchronize(s));
     uint64_t t = 0;
-    for (int i = 0; i < kPopSlots; ++i) t += h[(size_t)i * kPopSlotStride];
+    for (unsigned long long v : h) t += v;
     return t;
 }
 

@@ -278,6 +278,26 @@ def test_lazy_loads_and_agprs_grow_partitions_without_spilling():
     assert not tuning.register_clean(dict(info, agprs=0)) or all(k["agprs"] == 0 for k in info["kernels"])
 
 
+def test_join_warmup_is_its_own_kernel_and_compiles_clean():
+    """`join_warmup` changes the generated MatchStar kernel only where a
+    step spans the warm-up rows and the chunk, and the joined kernel keeps
+    every value in registers."""
+    from bitstreamzoo_amd import workloads
+    w = workloads.matchstar(64 * 64 * 8, seed=1)
+    prog, shape = api.Program(w.source), api.Case.from_bscase(w.cases[0])
+
+    def entry(o):
+        info = api.compile_plan_only(prog, shape, o)
+        (k,) = info["kernels"]
+        assert k["scratch_bytes"] == 0 and k["agprs"] == 0, k
+        return k["entry"]
+
+    spans = {"chunk_rows": 4, "rows_per_step": 8}
+    assert entry(spans) != entry(dict(spans, join_warmup=1))
+    short = {"chunk_rows": 4, "rows_per_step": 4}   # warm-up + chunk exceed the step
+    assert entry(short) == entry(dict(short, join_warmup=1))
+
+
 def test_refused_schedule_is_an_answer_of_the_compile_pool():
     """Compiling a schedule the library refuses in the worker pool returns
     the refusal instead of breaking the pool, and the pool keeps working."""

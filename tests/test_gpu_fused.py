@@ -21,8 +21,8 @@ TILES = [{"chunk_rows": 1, "rows_per_step": 1, "warmup_rows": 1},
          {"chunk_rows": 16, "rows_per_step": 4, "warmup_rows": 1},
          {"chunk_rows": 8, "rows_per_step": 4, "warmup_rows": 1, "max_ops": 3, "max_live": 4},
          # one step holds the warm-up rows and the chunk (their loads issued together)
-         {"chunk_rows": 4, "rows_per_step": 8, "warmup_rows": 1},
-         {"chunk_rows": 2, "rows_per_step": 8, "warmup_rows": 3}]
+         {"chunk_rows": 4, "rows_per_step": 8, "warmup_rows": 1, "join_warmup": 1},
+         {"chunk_rows": 2, "rows_per_step": 8, "warmup_rows": 3, "join_warmup": 1}]
 
 
 def run_plan(src, cases, opts, names, replay=0):

@@ -92,11 +92,11 @@ def test_cost_model_ranks_by_a_known_law():
 
 def test_trained_model_beats_the_default_schedule_on_held_out_programs():
     """The gain over the method without the model: the cost model, trained on
-    the committed autotuner measurements, picks each held-out generated
-    program's schedule from plan features alone (top prediction over 32
-    sampled candidates, no trial runs); together the picks run faster than
-    the untuned default schedule, and every pick gives the default
-    schedule's outputs."""
+    the committed autotuner measurements, ranks 32 sampled schedules of each
+    held-out generated program from plan features alone, and its pick (the
+    fastest of the three best-predicted schedules that compile within their
+    register file) runs faster than the untuned default schedule on every
+    program and in total; every pick gives the default schedule's outputs."""
     from bitstreamzoo_amd import api, costmodel, schedsearch, tuning
     X, ms = schedsearch.load_training_data()
     model = costmodel.CostModel(ctx(), seed=0)
@@ -104,23 +104,19 @@ def test_trained_model_beats_the_default_schedule_on_held_out_programs():
     cands = schedsearch.sample_space(32, 4321)
     specs = schedsearch.random_specs(3, 99, nodes=(800, 1600), log2_bytes=(20, 21))
     built = [sp.build() for sp in specs]
+    c = ctx()
     picks = []
     for p, case in built:
         F = costmodel.features_batch(api.Program(p.source), tuning.shape_case([case]), cands,
                                      (case.bitlength + 63) // 64, 1)
-        picks.append(cands[int(np.argmin(model.predict_ms(F)))])
-    tuning.precompile_many([(p.source, [case], o) for (p, case), o in zip(built, picks)] +
-                           [(p.source, [case], {}) for p, case in built], processes=16)
-    c = ctx()
+        picks.append(schedsearch.model_pick(c, p.source, case, cands, model.predict_ms(F)).options[0])
+    tuning.precompile_many([(p.source, [case], {}) for p, case in built], processes=16)
     t_pick = t_default = 0.0
+    slower = []
     for (p, case), o in zip(built, picks):
         res = tuning.Resident(c, [case])
         prog, shape = api.Program(p.source), tuning.shape_case([case])
-        try:
-            pl = api.Plan(c, prog, shape, o)
-        except api.BszError as e:
-            assert tuning.refused(e)
-            pl = api.Plan(c, prog, shape, {})
+        pl = api.Plan(c, prog, shape, o)
         d = api.Plan(c, prog, shape, {})
         res.bind(pl)
         res.bind(d)
@@ -130,9 +126,14 @@ def test_trained_model_beats_the_default_schedule_on_held_out_programs():
         c.sync()
         for out in p.outputs:
             assert np.array_equal(pl.download_output(0, out, nw), d.download_output(0, out, nw)), out
-        t_pick += tuning.time_runs(c, [pl], 5, 2)
-        t_default += tuning.time_runs(c, [d], 5, 2)
+        tp = tuning.time_runs(c, [pl], 10, 2)
+        td = tuning.time_runs(c, [d], 10, 2)
+        if tp >= td:
+            slower.append((o, tp, td))
+        t_pick += tp
+        t_default += td
         pl.close()
         d.close()
         res.free()
+    assert not slower, slower
     assert t_pick < t_default, (t_pick, t_default)
