@@ -77,12 +77,15 @@ class Measured:
 
 
 def measure(ctx: api.Context, source: str, case, candidates: list[dict], steps: int = 5, processes: int = 16,
-            log=None, allow_empty: bool = False) -> Measured:
-    """Compile every candidate (parallel processes), then check and time each."""
+            log=None, allow_empty: bool = False, shards: int = 0) -> Measured:
+    """Compile every candidate (parallel processes, each plan's kernels in
+    `shards` jobs), then check and time each."""
     out = Measured()
     t0 = time.perf_counter()
-    tuning.precompile(source, [case], candidates, processes=processes, log=log)
+    tuning.precompile(source, [case], candidates, processes=processes, shards=shards, log=log)
     out.compile_s = time.perf_counter() - t0
+    if log:
+        log(f"measure: {len(candidates)} candidates compiled in {out.compile_s:.0f} s; checking and timing them")
     t1 = time.perf_counter()
     prog = api.Program(source)
     shape = tuning.shape_case([case])
@@ -136,7 +139,7 @@ PICK_TOP = 3   # predicted-best candidates the model's pick compiles and times
 
 
 def model_pick(ctx: api.Context, source: str, case, candidates: list[dict], pred: np.ndarray,
-               processes: int = 16, top: int = PICK_TOP) -> "Measured":
+               processes: int = 16, top: int = PICK_TOP, log=None, shards: int = 0) -> "Measured":
     """The model's choice: compile and time the `top` best-predicted
     candidates that compile within their register file (a schedule the
     autotuner would accept; rejected ones are replaced by the next
@@ -148,7 +151,7 @@ def model_pick(ctx: api.Context, source: str, case, candidates: list[dict], pred
     while pos < len(order) and len(got.ms) < top:
         batch = [candidates[i] for i in order[pos:pos + top - len(got.ms)]]
         pos += len(batch)
-        m = measure(ctx, source, case, batch, processes=processes, allow_empty=True)
+        m = measure(ctx, source, case, batch, processes=processes, allow_empty=True, log=log, shards=shards)
         got.options += m.options
         got.ms += m.ms
         got.compile_s += m.compile_s
@@ -329,7 +332,9 @@ def evaluate_large(ctx: api.Context, model: costmodel.CostModel, nodes: int, log
     if log:
         log(f"large: {nodes} nodes, features and predictions for {len(candidates)} candidates in {predict_s:.1f} s")
     t2 = time.perf_counter()
-    pick = model_pick(ctx, p.source, case, candidates, pred, processes, top=1)
+    # a plan of this size compiles in 64 jobs per schedule, so progress is
+    # reported every few seconds
+    pick = model_pick(ctx, p.source, case, candidates, pred, processes, top=1, log=log, shards=64)
     model_s = predict_s + (time.perf_counter() - t2)
     pick_opts, pick_ms = pick.options[0], float(pick.ms[0])
     if log:
@@ -340,7 +345,7 @@ def evaluate_large(ctx: api.Context, model: costmodel.CostModel, nodes: int, log
               if candidates[i] not in (pick_opts, default)]
     trial = [pick_opts, default] + others[:max(0, judge - 2)]
     t3 = time.perf_counter()
-    m = measure(ctx, p.source, case, trial, processes=processes, log=log, allow_empty=True)
+    m = measure(ctx, p.source, case, trial, processes=processes, log=log, allow_empty=True, shards=64)
     tune_s = time.perf_counter() - t3
     by = {json.dumps(o, sort_keys=True): ms for o, ms in zip(m.options, m.ms)}
     best = min(m.ms) if m.ms else pick_ms

@@ -36,3 +36,31 @@ def test_cpu_baseline_reports_both_modes_and_the_host():
     assert cb["kind"] == "port" and cb["unit"] == "Gbit-ops/s" and cb["value"] > 0
     assert cb["best_cpu"]["value"] > 0 and cb["cores"] >= 1 and cb["cpu_model"]
     assert cb["port_over_reference_time"]["faithful"] is not None
+
+
+def test_dagprof_joins_a_kernel_trace_with_the_plan(tmp_path):
+    """The trace analysis sums per-kernel average durations, prices each
+    kernel at its planned bytes and separates the kernels whose chunks ran
+    again after their look-back (two kernels sharing generated code share
+    their dispatches' average)."""
+    import json
+    from bitstreamzoo_amd import dagprof
+    plan = {"bitlength": 64 * 1000, "n_chunks": 4, "rerun_chunks": [0, 3, 0],
+            "kernels": [{"name": "bsz_k0", "entry": "bsz_fk_a", "inputs": 2, "outputs": 1},
+                        {"name": "bsz_k1", "entry": "bsz_fk_b", "inputs": 1, "outputs": 1},
+                        {"name": "bsz_k2", "entry": "bsz_fk_a", "inputs": 2, "outputs": 1}]}
+    pj = tmp_path / "plan.json"
+    pj.write_text(json.dumps(plan))
+    head = "Kind,Kernel_Name,Start_Timestamp,End_Timestamp\n"
+    rows = ["KERNEL_DISPATCH,bsz_fk_a,0,1000", "KERNEL_DISPATCH,bsz_fk_a,2000,5000",
+            "KERNEL_DISPATCH,bsz_fk_b,6000,10000", "KERNEL_DISPATCH,k_other,0,99999"]
+    tr = tmp_path / "trace.csv"
+    tr.write_text(head + "\n".join(rows) + "\n")
+    r = dagprof.analyze(str(tr), str(pj))
+    sb = 8 * 1000
+    assert r["kernels"] == 3
+    assert r["kernel_ms_per_run"] == round((2000 + 4000 + 2000) / 1e6, 1)
+    assert r["planned_bytes"] == (3 + 2 + 3) * sb
+    assert r["kernels_with_reruns"]["count"] == 1 and r["kernels_with_reruns"]["chunks_rerun"] == 3
+    assert r["kernels_without_reruns"]["GBps"] == round(6 * sb / 4000, 1)
+    assert r["rate_GBps_quantiles"]["50"] == round(3 * sb / 2000, 1)
