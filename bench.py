@@ -220,11 +220,15 @@ def one_pass_floor(ctx, args, residents, bitlength):
     """ms per step of the cheapest one-launch pass that moves MatchStar's
     bytes: r = M & C over the same rotating input sets (two streams read, one
     written, no carry, no warm-up).  A program of this size cannot run
-    faster than one such launch; the best of a few tile shapes is kept."""
+    faster than one such launch; the best of a few tile shapes, with cached
+    and nontemporal loads and stores and with 16-byte lanes, is kept."""
     prog = api.Program(FLOOR_SRC)
     best = None
     for opts in ({"chunk_rows": 4, "rows_per_step": 4}, {"chunk_rows": 8, "rows_per_step": 8},
-                 {"chunk_rows": 16, "rows_per_step": 8}, {"chunk_rows": 16, "rows_per_step": 4, "interleave": 1}):
+                 {"chunk_rows": 16, "rows_per_step": 8}, {"chunk_rows": 16, "rows_per_step": 4, "interleave": 1},
+                 {"chunk_rows": 4, "rows_per_step": 4, "nontemporal": 2},
+                 {"chunk_rows": 8, "rows_per_step": 8, "nontemporal": 2},
+                 {"chunk_rows": 8, "rows_per_step": 8, "words_per_lane": 2, "nontemporal": 2}):
         plans = []
         for r in residents:
             p = api.Plan(ctx, prog, tuning.shape_case(r.cases), opts)
@@ -711,6 +715,7 @@ def main():
             ex[name]["one_launch_floor_ms"] = round(rr["floor_ms"], 5)
             ex[name]["one_launch_floor_hbm_frac"] = round(rr["alg_bytes"] / (rr["floor_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
             ex[name]["frac_of_floor"] = round(rr["floor_ms"] / m, 4)
+            ex[name]["one_launch_floor_schedule"] = rr["floor_options"]
     if ex:
         line["other_configs"] = ex
     if learned is not None:
