@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <queue>
 #include <set>
@@ -147,6 +148,14 @@ __device__ __forceinline__ u64 ldr(const GAS u64* rowbase, long long off, long l
 // (say `b ^ b`).  Every advance and sum is therefore passed through opq(), so
 // that no stream value carries a known half.
 __device__ __forceinline__ u64 opq(u64 x) { __asm__("" : "+v"(x)); return x; }
+// Any boolean function of three 64-bit words in two v_bitop3_b32 (one per
+// 32-bit half): bit i of the result is bit (a_i << 2 | b_i << 1 | c_i) of T.
+template <unsigned T>
+__device__ __forceinline__ u64 bop3(u64 a, u64 b, u64 c) {
+  const u32 lo = __builtin_amdgcn_bitop3_b32((u32)a, (u32)b, (u32)c, T);
+  const u32 hi = __builtin_amdgcn_bitop3_b32((u32)(a >> 32), (u32)(b >> 32), (u32)(c >> 32), T);
+  return opq(((u64)hi << 32) | lo);
+}
 __device__ __forceinline__ u64 fsl(u64 cur, u64 lower, u32 b) { return b ? ((cur << b) | (lower >> (64 - b))) : cur; }
 __device__ __forceinline__ u64 fsr(u64 cur, u64 upper, u32 b) { return b ? ((cur >> b) | (upper << (64 - b))) : cur; }
 __device__ __forceinline__ u32 wave_sum(u32 v) {
@@ -503,6 +512,78 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
     for (int j = 0; j < NOUT; ++j) out_of[ks.outputs[j]].push_back(j);
     for (int p = 0; p < NPOP; ++p) pop_of[g.pops[ks.pops[p]]].push_back(p);
 
+    // Three-input boolean functions.  A NOT/AND/OR/XOR node absorbs the
+    // boolean nodes of this kernel that feed only it (no other consumer, not
+    // stored, not counted) while its leaves stay within three values, and is
+    // emitted as one bop3 (two v_bitop3_b32) over the leaves with the truth
+    // table of the absorbed expression.  The absorbed nodes are still
+    // emitted; their values are then unused and the compiler drops them.
+    std::map<int32_t, std::pair<std::vector<int32_t>, unsigned>> fused3;   // node -> (leaves, table)
+    if (opt.bitop3) {
+        auto is_bool = [&](NOp op) { return op == NOp::Not || op == NOp::And || op == NOp::Or || op == NOp::Xor; };
+        std::map<int32_t, int> kuse;
+        for (int32_t n : ks.nodes) {
+            const GNode& x = g.nodes[n];
+            if (arity(x.op) >= 1 && in_kernel.count(x.a)) ++kuse[x.a];
+            if (arity(x.op) >= 2 && in_kernel.count(x.b)) ++kuse[x.b];
+        }
+        auto absorbable = [&](int32_t m) {
+            return in_kernel.count(m) && is_bool(g.nodes[m].op) && kuse[m] == 1 && !out_of.count(m) && !pop_of.count(m);
+        };
+        auto operands = [&](int32_t m) {
+            std::vector<int32_t> v;
+            const GNode& x = g.nodes[m];
+            if (arity(x.op) >= 1 && g.nodes[x.a].op != NOp::Zero) v.push_back(x.a);
+            if (arity(x.op) >= 2 && g.nodes[x.b].op != NOp::Zero) v.push_back(x.b);
+            return v;
+        };
+        for (int32_t n : ks.nodes) {
+            if (!is_bool(g.nodes[n].op)) continue;
+            std::set<int32_t> inl{n};
+            std::vector<int32_t> leaves = operands(n);
+            std::sort(leaves.begin(), leaves.end());
+            leaves.erase(std::unique(leaves.begin(), leaves.end()), leaves.end());
+            for (bool grew = true; grew;) {
+                grew = false;
+                for (int32_t l : leaves) {
+                    if (!absorbable(l)) continue;
+                    std::vector<int32_t> nl;
+                    for (int32_t y : leaves) if (y != l) nl.push_back(y);
+                    for (int32_t y : operands(l)) nl.push_back(y);
+                    std::sort(nl.begin(), nl.end());
+                    nl.erase(std::unique(nl.begin(), nl.end()), nl.end());
+                    if (nl.size() > 3) continue;
+                    inl.insert(l);
+                    leaves = nl;
+                    grew = true;
+                    break;
+                }
+            }
+            if (inl.size() < 2) continue;
+            // truth table: leaf j takes bit (2 - j) of the index
+            unsigned table = 0;
+            for (unsigned idx = 0; idx < 8; ++idx) {
+                std::function<bool(int32_t)> ev = [&](int32_t m) -> bool {
+                    for (size_t j = 0; j < leaves.size(); ++j)
+                        if (leaves[j] == m) return (idx >> (2 - j)) & 1u;
+                    const GNode& x = g.nodes[m];
+                    if (x.op == NOp::Zero) return false;
+                    switch (x.op) {
+                        case NOp::Not: return !ev(x.a);
+                        case NOp::And: return ev(x.a) && ev(x.b);
+                        case NOp::Or: return ev(x.a) || ev(x.b);
+                        case NOp::Xor: return ev(x.a) != ev(x.b);
+                        default: throw std::logic_error("internal: bop3 over a non-boolean node");
+                    }
+                };
+                if (ev(n)) table |= 1u << idx;
+            }
+            // a constant function would hand the compiler a known value
+            if (table == 0u || table == 0xffu) continue;
+            fused3[n] = {leaves, table};
+        }
+    }
+
     // Loads of U rows starting at row r (row u: `r + u`, or `r + u * istride`
     // when interleaved); row u exists while its index is below `limit`.
     // Row u's words of the kernel's inputs.  `guard`: the row may not exist
@@ -648,10 +729,24 @@ void gen_kernel(const SegGraph& g, const PlanOptions& opt, KernelSpec& ks, bool 
             switch (x.op) {
                 case NOp::Zero: s << "        const u64 " << v << " = 0ull;\n"; break;
                 case NOp::Ones: s << "        const u64 " << v << " = m;\n"; break;
-                case NOp::Not: s << "        const u64 " << v << " = ~" << ref(x.a) << ";\n"; break;
-                case NOp::And: s << "        const u64 " << v << " = " << ref(x.a) << " & " << ref(x.b) << ";\n"; break;
-                case NOp::Or: s << "        const u64 " << v << " = " << ref(x.a) << " | " << ref(x.b) << ";\n"; break;
-                case NOp::Xor: s << "        const u64 " << v << " = " << ref(x.a) << " ^ " << ref(x.b) << ";\n"; break;
+                case NOp::Not: case NOp::And: case NOp::Or: case NOp::Xor: {
+                    auto f3 = fused3.find(n);
+                    if (f3 != fused3.end()) {
+                        const std::vector<int32_t>& lv = f3->second.first;
+                        s << "        const u64 " << v << " = bop3<" << f3->second.second << "u>(";
+                        for (size_t j = 0; j < 3; ++j) {
+                            if (j < lv.size()) need_x(lv[j]);
+                            s << (j ? ", " : "") << (j < lv.size() ? ref(lv[j]) : std::string("0ull"));
+                        }
+                        s << ");\n";
+                    } else if (x.op == NOp::Not) {
+                        s << "        const u64 " << v << " = ~" << ref(x.a) << ";\n";
+                    } else {
+                        const char* o = x.op == NOp::And ? " & " : x.op == NOp::Or ? " | " : " ^ ";
+                        s << "        const u64 " << v << " = " << ref(x.a) << o << ref(x.b) << ";\n";
+                    }
+                    break;
+                }
                 case NOp::Add: {
                     const std::string st = "st" + std::to_string(state_of.at(n)), t = "_" + std::to_string(n);
                     s << "        const u64 a" << t << " = " << ref(x.a) << ", s" << t << " = a" << t << " + " << ref(x.b) << ";\n";
@@ -945,7 +1040,7 @@ std::string PlanOptions::key() const {
     std::ostringstream s;
     s << chunk_rows << "," << warmup_rows << "," << waves_per_block << "," << rows_per_step << "," << max_ops << ","
       << max_live << "," << occupancy << "," << words_per_lane << "," << store_order << "," << nontemporal << ","
-      << interleave << "," << persistent << "," << agprs << "," << lazy_loads << "," << join_warmup;
+      << interleave << "," << persistent << "," << agprs << "," << lazy_loads << "," << join_warmup << "," << bitop3;
     return s.str();
 }
 

@@ -88,6 +88,7 @@ struct PlanOptions {
     bool agprs = false;              // occupancy 1 only: values may also live in the 256 AGPRs
     bool lazy_loads = false;         // load each input word at its first use in a row, not at the row's start
     bool join_warmup = false;        // a stateful kernel whose step spans its chunk loads warm-up and chunk together
+    bool bitop3 = true;              // boolean nodes fused into three-input functions (v_bitop3_b32)
     std::string key() const;
 };
 

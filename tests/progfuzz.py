@@ -181,3 +181,32 @@ def random_case(seed: int, bitlength: int, density: float | None = None) -> BsCa
     c.inputs = {f"x{i}": stream() for i in range(3)}
     c.input_arrays = {"arr": {i: stream() for i in range(4)}}
     return c
+
+
+def bool_chain_program(seed: int, n_out: int = 24) -> str:
+    """Outputs that are chains of NOT/AND/OR/XOR over four inputs; every
+    temporary feeds the next link only (so the generator may fold a link into
+    a three-input function), except where a chain reuses an earlier
+    temporary as its other operand (so it may not)."""
+    rng = np.random.default_rng(seed)
+    lines = ["input stream a", "input stream b", "input stream c", "input stream d"]
+    lines += [f"output stream r{i}" for i in range(n_out)] + ["output int n0"]
+    t = 0
+    for i in range(n_out):
+        cur, made = str(rng.choice(["a", "b", "c", "d"])), []
+        for _ in range(int(rng.integers(1, 6))):
+            t += 1
+            op = str(rng.choice(["&", "|", "^", "~"]))
+            if op == "~":
+                lines.append(f"stream t{t} = ~{cur}")
+            else:
+                pool = ["a", "b", "c", "d"] + (made[:-1] if rng.random() < 0.3 else [])
+                other = str(rng.choice(pool))
+                lines.append(f"stream t{t} = {cur} {op} {other}" if rng.random() < 0.5 else
+                             f"stream t{t} = {other} {op} {cur}")
+            made.append(f"t{t}")
+            cur = f"t{t}"
+        lines.append(f"r{i} = {cur}")
+    lines.append("n0 = popcount(r0)")
+    return "\n".join(lines) + "\n"
+

@@ -298,6 +298,21 @@ def test_join_warmup_is_its_own_kernel_and_compiles_clean():
     assert entry(short) == entry(dict(short, join_warmup=1))
 
 
+def test_boolean_chains_fold_into_three_input_functions():
+    """With `bitop3` (the default) single-use boolean links fold into
+    three-input functions: the generated kernel differs from the unfolded
+    one, compiles within its registers, and needs no more of them."""
+    src = progfuzz.bool_chain_program(0)
+    case = BsCase(bitlength=64, inputs={n: np.zeros(1, np.uint64) for n in "abcd"})
+    prog, shape = api.Program(src), api.Case.from_bscase(case)
+    on = api.compile_plan_only(prog, shape, {"chunk_rows": 4})["kernels"]
+    off = api.compile_plan_only(prog, shape, {"chunk_rows": 4, "bitop3": 0})["kernels"]
+    assert len(on) == len(off) == 1
+    assert on[0]["entry"] != off[0]["entry"]
+    assert on[0]["scratch_bytes"] == 0 and on[0]["agprs"] == 0
+    assert on[0]["vgprs"] <= off[0]["vgprs"]
+
+
 def test_refused_schedule_is_an_answer_of_the_compile_pool():
     """Compiling a schedule the library refuses in the worker pool returns
     the refusal instead of breaking the pool, and the pool keeps working."""

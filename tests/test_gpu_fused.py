@@ -282,3 +282,17 @@ def test_random_straight_line_plans_interleaved(seed):
     ora, _ = run_model(Oracle, src, case, Oracle.SCALAR, stream_names=names[0], int_names=names[1])
     got, _ = run_plan(src, [case], dict(INTERLEAVED[seed % len(INTERLEAVED)], max_ops=8), names)
     assert_same(got[0], ora, case.bitlength, set(names[1]), f"seed {seed}")
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("bitop3", [1, 0])
+def test_three_input_boolean_functions(seed, bitop3):
+    """Boolean chains folded into three-input functions (v_bitop3_b32) give
+    the oracle's words, and so do the same chains without the folding."""
+    src = progfuzz.bool_chain_program(seed)
+    bl = 64 * 64 * 5 + 17
+    rng = np.random.default_rng(100 + seed)
+    case = BsCase(bitlength=bl, inputs={n: workloads.random_words(rng, bl) for n in "abcd"})
+    ora, names = oracle_outs(src, case)
+    got, _ = run_plan(src, [case], {"bitop3": bitop3, "chunk_rows": 4}, names)
+    assert_same(got[0], ora, bl, set(names[1]), f"boolean chains seed {seed} bitop3 {bitop3}")
