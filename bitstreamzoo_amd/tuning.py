@@ -224,8 +224,11 @@ def default_space(kind: str = "auto") -> list[dict]:
         # one kernel per program where it fits (every split into smaller
         # kernels, with or without 16-byte lanes, measured slower or equal
         # on the 65-input adder while moving 2 % more bytes); nontemporal
-        # loads and stores (2) measured 5 % faster on that adder
-        grid = {"chunk_rows": [16, 32, 64], "rows_per_step": [1, 2], "occupancy": [1, 2],
+        # loads and stores (2) measured 5 % faster on that adder.  64-row
+        # chunks leave one wave per SIMD on that adder, and their time moved
+        # by 8 % between processes of the same run (0.599 against 0.646 ms),
+        # so they are not candidates.
+        grid = {"chunk_rows": [16, 32], "rows_per_step": [1, 2], "occupancy": [1, 2],
                 "waves_per_block": [4], "interleave": [0, 1], "nontemporal": [0, 2]}
     elif kind == "stateful":
         # nontemporal loads and stores (2) measured 15 % faster on MatchStar
