@@ -1,8 +1,9 @@
 // fused.cpp — the fused executor: segment recording, plan cache, hipRTC
 // compilation with an on-disk code-object cache, plan launch (per partition
-// one kernel whose wave w owns chunk w of the batch, plus a one-workgroup
-// fix-up launch for kernels that carry state across chunks; optionally
-// captured into a hipGraph) and the batched plan interface.
+// one kernel and one launch per run, wave w owning chunk w of the batch; a
+// chunk whose warm-up leaves its start state unproven looks back at its
+// predecessor's end state inside the same launch; optionally captured into
+// a hipGraph) and the batched plan interface.
 #include "fused.h"
 
 #include "bsdata_io.h"

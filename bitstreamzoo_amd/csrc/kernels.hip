@@ -293,8 +293,7 @@ __global__ __launch_bounds__(256) void k_tail(uint64_t* dst, uint64_t k, uint64_
     if (blockIdx.x == 0 && threadIdx.x == 0 && lw >= 1 && lw - 1 < k) dst[lw - 1] &= lastmask;
 }
 
-// Ends a fused plan run whose last kernel has no fix-up launch: the run
-// counter selects the popcount bank the next run adds into.
+// Copies n_pairs 16-byte word pairs.
 __global__ __launch_bounds__(256) void k_copy_words(uint64_t* __restrict__ dst, const uint64_t* __restrict__ src,
                                                     uint64_t n_pairs) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

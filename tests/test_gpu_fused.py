@@ -1,6 +1,7 @@
 """Fused plans (bsz_plan_*) against the oracle: tile shapes that cut streams
-into many chunks, carry chains that defeat the speculative warm-up so the
-in-order fix-up has to recompute chunks, advances of in-kernel values across
+into many chunks, carry chains that defeat the speculative warm-up so chunks
+look back at their predecessor's end state and run again, three-input
+boolean folding, advances of in-kernel values across
 rows and chunks, multi-kernel plans with intermediate buffers, batches of
 instances of uneven lengths in one launch, and hipGraph replay."""
 

@@ -5,8 +5,8 @@ A generated kernel runs `warmup_rows` rows before its chunk from a zero
 state.  Per row it tracks, for every value, the lowest lane L from which the
 value is certainly exact, and for every state word (an add's carry, an
 advance's previous lane-63 word) whether it is exact.  A chunk whose states
-are all proven exact skips the fix-up launch, so a rule that claims too much
-would leave wrong bits.  Here random small DAGs over 64-lane rows (one row
+are all proven exact does not look back at its predecessor's end state, so
+a rule that claims too much would leave wrong bits.  Here random small DAGs over 64-lane rows (one row
 is a 4096-bit integer) run twice, from the true state and from the zero
 state, and every claim of the rules is compared with the true run.  The same
 check with a deliberately weakened rule must find a violation."""
