@@ -362,6 +362,10 @@ def bench_learned(ctx, args):
 def bench_mixed(ctx, args, full: bool = True):
     """A mixed batch of generated regex-style DAG programs, each over its own
     text, all run per step as one captured hipGraph of every plan's launches.
+    The programs are spread over `--mixed-streams` streams (one context each,
+    balanced by ops x bits), which fork from the capturing stream and join it
+    again, so independent programs run concurrently; the same graph with the
+    streams chained one after another is timed beside it.
     Each program's schedule is the learned model's top prediction over 64
     sampled candidates (no trial runs); every program's outputs are checked
     against the same program under the default schedule, and the measured
@@ -398,24 +402,28 @@ def bench_mixed(ctx, args, full: bool = True):
     log(f"mixed: {n} programs, picks in {pick_s:.1f} s, compiled in {compile_s:.1f} s")
     residents, plans, defaults = [], [], []
     bit_ops = alg = launches = refused = 0
+    n_streams = max(1, args.mixed_streams)
+    ctxs = [ctx] + [api.Context(0) for _ in range(n_streams - 1)]
+    bins = tuning.balance([p.n_dsl_ops * case.bitlength for p, case in built], n_streams)
     try:
         for i, ((p, case), o) in enumerate(zip(built, picks)):
-            r = tuning.Resident(ctx, [case])
+            c = ctxs[bins[i]]
+            r = tuning.Resident(c, [case])
             residents.append(r)
             prog = api.Program(p.source)
             try:
-                pl = api.Plan(ctx, prog, tuning.shape_case([case]), o)
+                pl = api.Plan(c, prog, tuning.shape_case([case]), o)
             except api.BszError as e:
                 if not tuning.refused(e):
                     raise
                 # the model picked a schedule whose kernels would overflow
                 refused += 1
                 picks[i] = {}
-                pl = api.Plan(ctx, prog, tuning.shape_case([case]), {})
+                pl = api.Plan(c, prog, tuning.shape_case([case]), {})
             r.bind(pl)
             plans.append(pl)
             if i in checked:
-                d = api.Plan(ctx, prog, tuning.shape_case([case]), {})
+                d = api.Plan(c, prog, tuning.shape_case([case]), {})
                 r.bind(d)
                 defaults.append((i, d))
             bit_ops += p.n_dsl_ops * case.bitlength
@@ -424,22 +432,29 @@ def bench_mixed(ctx, args, full: bool = True):
         for i, d in defaults:
             plans[i].run()
             d.run()
-        ctx.sync()
+        for c in ctxs:
+            c.sync()
         for i, d in defaults:
             p, case = built[i]
             nw = (case.bitlength + 63) // 64
             for o in p.outputs:
                 if not np.array_equal(plans[i].download_output(0, o, nw), d.download_output(0, o, nw)):
                     raise SystemExit(f"mixed: output {o} of program {i} differs between two schedules")
-        ms = time_plans(ctx, plans, args.steps, args.warmup, api.FUSED) * len(plans)
-        default_ms = (time_plans(ctx, [d for _, d in defaults], args.steps, args.warmup, api.FUSED) * len(defaults)
-                      if full else None)
+        def groups(items):
+            return [(ctxs[k], [pl for i, pl in items if bins[i] == k]) for k in range(n_streams)]
+        ms = tuning.time_runs_streams(ctx, groups(list(enumerate(plans))), args.steps, args.warmup)
+        serial_ms = tuning.time_runs_streams(ctx, groups(list(enumerate(plans))), args.steps, args.warmup,
+                                             serial=True)
+        default_ms = (tuning.time_runs_streams(ctx, groups(defaults), args.steps, args.warmup) if full else None)
     finally:
         for pl in plans + [d for _, d in defaults]:
             pl.close()
         for r in residents:
             r.free()
-    log(f"mixed: batch {ms:.3f} ms/step with the model's schedules ({launches} launches per step); "
+        for c in ctxs[1:]:
+            c.close()
+    log(f"mixed: batch {ms:.3f} ms/step with the model's schedules on {n_streams} streams "
+        f"({serial_ms:.3f} ms with the streams one after another; {launches} launches per step); "
         f"default schedules {default_ms} ms; {len(defaults)} programs checked against their default schedule")
     judged = []
     sub = [cands[i] for i in sorted(rng.choice(len(cands), size=16, replace=False).tolist())]
@@ -455,6 +470,7 @@ def bench_mixed(ctx, args, full: bool = True):
                        "gap_pct": round(100 * (pick_ms / float(min(m.ms)) - 1), 2)})
         log(f"mixed: judged program {i}: pick {pick_ms:.3f} ms, autotuned best {min(m.ms):.3f} ms")
     return {"programs": n, "nodes": [lo, hi], "log2_bytes": [blo, bhi], "ms_per_step": ms, "refused_picks": refused,
+            "streams": n_streams, "streams_in_series_ms_per_step": serial_ms,
             "checked_against_default": len(defaults),
             "default_ms_per_step": default_ms, "launches_per_step": launches, "bit_ops": bit_ops,
             "alg_bytes": alg, "pick_s": round(pick_s, 2), "compile_s": round(compile_s, 1), "judged": judged}
@@ -569,6 +585,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner; use fixed default schedules")
     ap.add_argument("--schedule", default="", help="JSON options for the measured workload's plan (skips tuning)")
     ap.add_argument("--mixed-programs", type=int, default=256, help="programs of the mixed batch (0 skips it)")
+    ap.add_argument("--mixed-streams", type=int, default=8, help="streams the mixed batch's programs run on")
     ap.add_argument("--mixed-nodes", default="200,2000", help="generator nodes of the mixed batch, low,high")
     ap.add_argument("--mixed-log2-bytes", default="16,20", help="text bytes (log2) of the mixed batch, low,high")
     ap.add_argument("--mixed-judged", type=int, default=2, help="mixed-batch programs re-tuned by the autotuner")
@@ -708,7 +725,8 @@ def main():
             ex[name]["outputs_checked_against_perop"] = rr["outputs_checked"]
         if name == "mixed":
             for k in ("programs", "nodes", "log2_bytes", "default_ms_per_step", "launches_per_step", "pick_s",
-                      "compile_s", "refused_picks", "checked_against_default", "judged"):
+                      "compile_s", "refused_picks", "checked_against_default", "judged", "streams",
+                      "streams_in_series_ms_per_step"):
                 ex[name][k] = rr[k]
         if "floor_ms" in rr:
             # the same bytes in the cheapest single launch (r = M & C)

@@ -90,6 +90,7 @@ _SIGS = {
     "bsz_timer_stop": (C.c_int, [vp, C.POINTER(C.c_double)]),
     "bsz_capture_begin": (C.c_int, [vp]),
     "bsz_capture_end": (C.c_int, [vp, C.POINTER(vp)]),
+    "bsz_stream_order": (C.c_int, [vp, vp]),
     "bsz_graph_launch": (C.c_int, [vp, vp, C.c_int]),
     "bsz_graph_destroy": (C.c_int, [vp]),
     "bsz_mlp_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),

@@ -39,6 +39,11 @@ class DevStream:
         check(lib().bsz_d2h(self.ctx.h, ptr(out), self.dev, n), self.ctx.h)
         return out
 
+    def write(self, words: np.ndarray):
+        """Overwrite the stream with host words (truncated to the bitlength)."""
+        w = as_u64(words)
+        check(lib().bsz_h2d(self.ctx.h, self.dev, ptr(w), len(w), self.bitlength), self.ctx.h)
+
     def free(self):
         if self.dev is not None:
             check(lib().bsz_buf_free(self.ctx.h, self.dev), self.ctx.h)
@@ -76,6 +81,13 @@ class Context:
         v = C.c_double()
         check(lib().bsz_timer_stop(self.h, C.byref(v)), self.h)
         return v.value
+
+    def order_after(self, other: "Context"):
+        """Work enqueued on this context from now on runs after the work
+        enqueued so far on `other` (inside a capture on `other`: forks the
+        capture into this context's stream; `other.order_after(self)` joins
+        it back)."""
+        check(lib().bsz_stream_order(other.h, self.h), self.h)
 
     def capture(self, fn) -> "Graph":
         """Capture the launches `fn()` enqueues on this context into a graph."""

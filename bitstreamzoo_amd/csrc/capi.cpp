@@ -26,6 +26,7 @@ struct bsz_ctx {
     size_t add_state_bytes = 0;
     unsigned long long* slot = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipEvent_t order_ev = nullptr;   // recorded by bsz_stream_order on this context's stream
     bool capturing = false;          // between bsz_capture_begin and bsz_capture_end
     int64_t capture_launch0 = 0;     // kernel_launch_count() at bsz_capture_begin
 };
@@ -131,6 +132,7 @@ int bsz_ctx_destroy(bsz_ctx* ctx) {
         if (ctx->add_state) (void)hipFree(ctx->add_state);
         if (ctx->slot) (void)hipFree(ctx->slot);
         if (ctx->t0) { (void)hipEventDestroy(ctx->t0); (void)hipEventDestroy(ctx->t1); }
+        if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
         (void)hipStreamDestroy(ctx->s);
     });
     delete ctx;
@@ -509,6 +511,18 @@ int bsz_capture_end(bsz_ctx* ctx, void** graph_exec) {
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         *graph_exec = ex;
+    });
+}
+
+int bsz_stream_order(bsz_ctx* from, bsz_ctx* to) {
+    return guard(to, [&] {
+        if (!from || from == to) throw std::invalid_argument("bsz_stream_order: needs two different contexts");
+        if (from->device != to->device) throw std::invalid_argument("bsz_stream_order: contexts on different devices");
+        if (!from->order_ev) BSZ_HIP(hipEventCreateWithFlags(&from->order_ev, hipEventDisableTiming));
+        // the wait takes the event as recorded at this point; a later record
+        // on `from` does not change what `to` waits for
+        BSZ_HIP(hipEventRecord(from->order_ev, from->s));
+        BSZ_HIP(hipStreamWaitEvent(to->s, from->order_ev, 0));
     });
 }
 

@@ -136,6 +136,63 @@ def shape_case(cases: list[BsCase]) -> api.Case:
     return api.Case.from_bscase(small)
 
 
+def time_runs_streams(ctx: api.Context, groups: list, steps: int, warmup: int, serial: bool = False) -> float:
+    """ms per batch, a batch being one run of every plan of `groups`, a list
+    of (context, plans) with one context per stream (`ctx` may hold a group
+    of its own).  The batch is captured on `ctx`'s stream: the other streams
+    fork from it at the start and join it at the end, so the groups run
+    concurrently inside one hipGraph; with `serial` each group starts after
+    the previous one ends (the same graph shape, one group at a time).
+    Replayed and timed with hipEvents on `ctx`'s stream."""
+    for c, plans in groups:
+        for p in plans:
+            p.set_executor(api.FUSED)
+            p.prepare()
+            p.run()
+    for c, _ in groups:
+        c.sync()
+    others = [c for c, _ in groups if c is not ctx]
+
+    def batch():
+        if serial:
+            prev = ctx
+            for c, plans in groups:
+                if c is not prev:
+                    c.order_after(prev)
+                for p in plans:
+                    p.run()
+                prev = c
+        else:
+            for c in others:
+                c.order_after(ctx)
+            for c, plans in groups:
+                for p in plans:
+                    p.run()
+        for c in others:
+            ctx.order_after(c)
+
+    g = ctx.capture(batch)
+    g.launch(warmup)
+    ctx.sync()
+    ctx.timer_start()
+    g.launch(steps)
+    ms = ctx.timer_stop() / steps
+    del g
+    return ms
+
+
+def balance(costs: list[float], n_bins: int) -> list[int]:
+    """Bin of each item: longest-processing-time first, each to the bin with
+    the least cost so far."""
+    load = [0.0] * n_bins
+    out = [0] * len(costs)
+    for i in sorted(range(len(costs)), key=lambda i: -costs[i]):
+        b = min(range(n_bins), key=lambda k: load[k])
+        out[i] = b
+        load[b] += costs[i]
+    return out
+
+
 class Resident:
     """Every case's inputs uploaded once to device streams."""
 

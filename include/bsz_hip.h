@@ -210,6 +210,15 @@ int bsz_timer_start(bsz_ctx* ctx);
 int bsz_timer_stop(bsz_ctx* ctx, double* ms);
 int bsz_capture_begin(bsz_ctx* ctx);
 int bsz_capture_end(bsz_ctx* ctx, void** graph_exec);
+/* Orders the work enqueued after this call on `to`'s stream after the work
+ * enqueued so far on `from`'s stream (an event recorded on `from`, waited on
+ * by `to`; the host does not wait).  While `from` captures, this forks the
+ * capture into `to`'s stream, and a later bsz_stream_order(to, from) joins
+ * it back before bsz_capture_end(from): independent plans bound to
+ * contexts of one device then run concurrently inside one graph.  No
+ * counterpart in the reference, which runs one process per input file on
+ * one stream (base.py:228-254). */
+int bsz_stream_order(bsz_ctx* from, bsz_ctx* to);
 int bsz_graph_launch(bsz_ctx* ctx, void* graph_exec, int times);
 int bsz_graph_destroy(void* graph_exec);
 
