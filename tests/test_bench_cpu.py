@@ -64,3 +64,18 @@ def test_dagprof_joins_a_kernel_trace_with_the_plan(tmp_path):
     assert r["kernels_with_reruns"]["count"] == 1 and r["kernels_with_reruns"]["chunks_rerun"] == 3
     assert r["kernels_without_reruns"]["GBps"] == round(6 * sb / 4000, 1)
     assert r["rate_GBps_quantiles"]["50"] == round(3 * sb / 2000, 1)
+
+
+def test_balance_spreads_programs_over_streams():
+    """Longest first, each to the least-loaded stream: the loads differ by
+    at most the largest single item, and every item gets a stream."""
+    import numpy as np
+    from bitstreamzoo_amd import tuning
+    rng = np.random.default_rng(3)
+    costs = list(rng.lognormal(0, 1.5, size=257))
+    for n in (1, 3, 8):
+        b = tuning.balance(costs, n)
+        assert len(b) == len(costs) and set(b) <= set(range(n))
+        load = [sum(c for c, k in zip(costs, b) if k == j) for j in range(n)]
+        assert max(load) - min(load) <= max(costs) + 1e-9
+    assert tuning.balance([5.0, 1.0, 1.0, 1.0, 1.0, 1.0], 2) == [0, 1, 1, 1, 1, 1]
