@@ -313,6 +313,40 @@ def test_boolean_chains_fold_into_three_input_functions():
     assert on[0]["vgprs"] <= off[0]["vgprs"]
 
 
+BOOL3 = [  # (DSL body over inputs a, b, c into r, the same function in Python)
+    ("stream t = a & b\nr = t | c\n", lambda a, b, c: (a & b) | c),
+    ("stream t = a ^ b\nr = t ^ c\n", lambda a, b, c: a ^ b ^ c),
+    ("stream t = a | b\nstream u = ~t\nr = u & c\n", lambda a, b, c: (1 - (a | b)) & c),
+    ("stream nb = ~b\nstream t = a & nb\nr = t ^ c\n", lambda a, b, c: (a & (1 - b)) ^ c),
+    ("stream t = c | a\nr = b & t\n", lambda a, b, c: b & (c | a)),
+]
+
+
+@pytest.mark.parametrize("body,fn", BOOL3)
+def test_three_input_truth_tables(body, fn, tmp_path, monkeypatch):
+    """The output's `bop3<T>` carries the truth table of the folded
+    expression: bit (a << 2 | b << 1 | c) of T, the operands passed in that
+    order (the operand order of v_bitop3_b32), as an independent evaluation
+    of the expression gives it.  Leaves are ordered by graph node, and an
+    input becomes a node at its first use, so a dead first statement reads
+    a, b, c in turn; the partition drops it."""
+    import glob
+    import re
+    monkeypatch.setenv("BSZ_DUMP_DIR", str(tmp_path))
+    src = ("input stream a\ninput stream b\ninput stream c\noutput stream r\n"
+           "stream z0 = a & b\nstream z1 = z0 & c\n" + body)
+    case = BsCase(bitlength=64, inputs={n: np.zeros(1, np.uint64) for n in "abc"})
+    api.compile_plan_only(api.Program(src), api.Case.from_bscase(case), {"chunk_rows": 1, "rows_per_step": 1})
+    (unit,) = [open(f).read() for f in glob.glob(str(tmp_path / "*.hip"))]
+    (v,) = set(re.findall(r"lane_st\(urow_w\(out0 \+ ru0 \* 64ull\), off8, (v\d+) & m\)", unit))
+    (call,) = set(re.findall(r"const u64 " + v + r" = bop3<(\d+)u>\((\w+?)_0, (\w+?)_0, (\w+?)_0\);", unit))
+    # the operands are the three inputs' words, in the order a, b, c
+    loads = {x: int(j) for x, j in re.findall(r"const u64 (x\d+)_0 = .*lane_ld\(urow\(in(\d+) ", unit)}
+    assert [loads[x] for x in call[1:]] == [0, 1, 2], (call, loads)
+    want = sum(1 << i for i in range(8) if fn((i >> 2) & 1, (i >> 1) & 1, i & 1))
+    assert int(call[0]) == want, (call, want)
+
+
 def test_refused_schedule_is_an_answer_of_the_compile_pool():
     """Compiling a schedule the library refuses in the worker pool returns
     the refusal instead of breaking the pool, and the pool keeps working."""
