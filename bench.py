@@ -593,8 +593,8 @@ def main():
                     help="held-out programs for the learned-vs-autotuned comparison (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the executor runs on a single MI355X and this build has no "
-              "multi-GPU path (see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the executor runs on a single MI355X and has no multi-GPU "
+              "path. Run with --gpus 1.", file=sys.stderr)
         return 2
 
     ctx = api.Context(0)

@@ -2,7 +2,7 @@
 profiler to watch, and write the plan's per-kernel shapes next to it.
 
     python -m bitstreamzoo_amd.dagprof --nodes 20000 --log2-bytes 27 \\
-        --schedule '{"warmup_rows": 4}' --out gpurun_dir/plan.json
+        --schedule '{"warmup_rows": 4}' --out plan.json
 
 The JSON holds every kernel's declared input and output stream counts (the
 bytes a kernel must move: 8 * words per stream), the chunk count and the

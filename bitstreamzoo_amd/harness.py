@@ -5,7 +5,8 @@
 benchmark/base.py:181-254): it runs the `bsim_hip` executable once per case
 (JSON on stdin) or per `.bsdata` file, and turns its JSON back into Python
 ints.  Registering it as backend "hip" in the reference's
-benchmark/run_all.py:make_backend is a two-line change.
+benchmark/run_all.py:make_backend is a two-line change: construct
+`HipBackend` where that function maps the name "hip".
 
 Everything here is process plumbing; the program runs in libbsz_hip.so on
 the GPU.
