@@ -79,3 +79,60 @@ def test_balance_spreads_programs_over_streams():
         load = [sum(c for c, k in zip(costs, b) if k == j) for j in range(n)]
         assert max(load) - min(load) <= max(costs) + 1e-9
     assert tuning.balance([5.0, 1.0, 1.0, 1.0, 1.0, 1.0], 2) == [0, 1, 1, 1, 1, 1]
+
+
+def test_stream_batch_forks_and_joins_every_stream():
+    """The captured batch forks every other stream from the capturing one
+    before any plan runs and joins each back at the end; `serial` chains the
+    streams instead, each starting after the previous group.  Checked on
+    stand-in contexts and plans that record the calls."""
+    from bitstreamzoo_amd import tuning
+
+    calls = []
+
+    class FakePlan:
+        def __init__(self, name):
+            self.name = name
+
+        def set_executor(self, e):
+            pass
+
+        def prepare(self):
+            pass
+
+        def run(self):
+            calls.append(("run", self.name))
+
+    class FakeGraph:
+        def launch(self, n):
+            pass
+
+    class FakeCtx:
+        def __init__(self, name):
+            self.name = name
+
+        def sync(self):
+            pass
+
+        def order_after(self, other):
+            calls.append(("order", other.name, self.name))
+
+        def capture(self, fn):
+            calls.clear()
+            fn()
+            return FakeGraph()
+
+        def timer_start(self):
+            pass
+
+        def timer_stop(self):
+            return 6.0
+
+    c0, c1, c2 = FakeCtx("c0"), FakeCtx("c1"), FakeCtx("c2")
+    groups = [(c0, [FakePlan("p0")]), (c1, [FakePlan("p1"), FakePlan("p2")]), (c2, [FakePlan("p3")])]
+    assert tuning.time_runs_streams(c0, groups, 3, 1) == 2.0
+    assert calls == [("order", "c0", "c1"), ("order", "c0", "c2"), ("run", "p0"), ("run", "p1"), ("run", "p2"),
+                     ("run", "p3"), ("order", "c1", "c0"), ("order", "c2", "c0")]
+    tuning.time_runs_streams(c0, groups, 3, 1, serial=True)
+    assert calls == [("run", "p0"), ("order", "c0", "c1"), ("run", "p1"), ("run", "p2"), ("order", "c1", "c2"),
+                     ("run", "p3"), ("order", "c1", "c0"), ("order", "c2", "c0")]
