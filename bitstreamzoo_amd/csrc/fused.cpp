@@ -84,6 +84,7 @@ FusedOptions FusedOptions::from_json_obj(const Json& j) {
     get("lazy_loads", o.lazy_loads);
     get("join_warmup", o.join_warmup);
     get("bitop3", o.bitop3);
+    get("order", o.order);
     get("compile", o.compile);
     get("compile_shard", o.compile_shard);
     get("compile_shards", o.compile_shards);
@@ -111,6 +112,7 @@ PlanOptions to_plan_options(const FusedOptions& f) {
     p.lazy_loads = f.lazy_loads != 0;
     p.join_warmup = f.join_warmup != 0;
     p.bitop3 = f.bitop3 != 0;
+    p.order = f.order == 1 ? 1 : 0;
     return p;
 }
 

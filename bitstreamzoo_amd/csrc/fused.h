@@ -29,6 +29,8 @@ struct FusedOptions {
     int join_warmup = 0;        // 1: a stateful kernel whose step spans its chunk loads the warm-up rows and the
                                 //   chunk in one step; 0: the warm-up rows run in a loop of their own first
     int bitop3 = 1;             // 1: chains of NOT/AND/OR/XOR fused into three-input functions (v_bitop3_b32)
+    int order = 0;              // partition order: 0 the recorded order, 1 a list schedule that keeps few
+                                //   values live (programs without device-side `if` selections)
     std::string cache_dir;      // code-object cache ("" = $BSZ_CACHE_DIR or ~/.cache/bsz_hip)
     int compile = 1;            // compile-only: 0 partitions and generates without running hipRTC
     int compile_shard = 0;      // compile-only: compile the kernels k with k % compile_shards == compile_shard

@@ -1041,7 +1041,104 @@ std::string PlanOptions::key() const {
     s << chunk_rows << "," << warmup_rows << "," << waves_per_block << "," << rows_per_step << "," << max_ops << ","
       << max_live << "," << occupancy << "," << words_per_lane << "," << store_order << "," << nontemporal << ","
       << interleave << "," << persistent << "," << agprs << "," << lazy_loads << "," << join_warmup << "," << bitop3;
+    if (order) s << ",o" << order;  // keys of recorded-order plans stay as they were
     return s.str();
+}
+
+// Partition order that keeps few values live (option order 1).  A list
+// schedule over the needed nodes: among the nodes whose operands are all
+// placed, take the one that ends the most live ranges (operands it is the
+// last unplaced consumer of) net of the value it starts, the most recently
+// readied first on ties, so a value's consumers tend to follow it while it is
+// still in the kernel being filled.  Inputs and constants are loaded, not
+// held, and are placed first.  Returns a topological order of all N nodes;
+// nodes that are not needed come last (the partitioner skips them).
+static std::vector<int32_t> pressure_order(const SegGraph& g, const std::vector<char>& needed) {
+    const int32_t N = (int32_t)g.nodes.size();
+    auto held = [&](int32_t a) { return a >= 0 && needed[a] && !is_free(g.nodes[a].op); };
+    auto operands = [&](int32_t n, int32_t (&o)[2]) {
+        const GNode& x = g.nodes[n];
+        int m = 0;
+        if (arity(x.op) >= 1 && held(x.a)) o[m++] = x.a;
+        if (arity(x.op) >= 2 && held(x.b) && x.b != x.a) o[m++] = x.b;
+        return m;
+    };
+    // consumers of each node (CSR)
+    std::vector<int32_t> cstart(N + 1, 0), cons;
+    for (int32_t n = 0; n < N; ++n) {
+        if (!held(n)) continue;
+        int32_t o[2];
+        const int m = operands(n, o);
+        for (int i = 0; i < m; ++i) ++cstart[o[i] + 1];
+    }
+    for (int32_t n = 0; n < N; ++n) cstart[n + 1] += cstart[n];
+    cons.resize(cstart[N]);
+    {
+        std::vector<int32_t> fill(cstart.begin(), cstart.end() - 1);
+        for (int32_t n = 0; n < N; ++n) {
+            if (!held(n)) continue;
+            int32_t o[2];
+            const int m = operands(n, o);
+            for (int i = 0; i < m; ++i) cons[fill[o[i]]++] = n;
+        }
+    }
+    std::vector<int32_t> rem(N), waiting(N, 0), ready_at(N, 0);
+    for (int32_t n = 0; n < N; ++n) {
+        rem[n] = cstart[n + 1] - cstart[n];
+        int32_t o[2];
+        if (held(n)) waiting[n] = operands(n, o);
+    }
+    std::vector<char> placed(N, 0);
+    std::vector<int32_t> order;
+    order.reserve(N);
+    for (int32_t n = 0; n < N; ++n)
+        if (needed[n] && is_free(g.nodes[n].op)) { order.push_back(n); placed[n] = 1; }
+    auto score = [&](int32_t n) {
+        int32_t o[2];
+        const int m = operands(n, o);
+        int s = rem[n] > 0 ? -1 : 0;
+        for (int i = 0; i < m; ++i) s += rem[o[i]] == 1;
+        return s;
+    };
+    struct Entry { int score; int32_t ready; int32_t node; };
+    auto worse = [](const Entry& a, const Entry& b) {
+        if (a.score != b.score) return a.score < b.score;
+        if (a.ready != b.ready) return a.ready < b.ready;
+        return a.node > b.node;
+    };
+    std::priority_queue<Entry, std::vector<Entry>, decltype(worse)> pq(worse);
+    for (int32_t n = 0; n < N; ++n)
+        if (held(n) && waiting[n] == 0) pq.push({score(n), 0, n});
+    int32_t t = 0;
+    while (!pq.empty()) {
+        const Entry e = pq.top();
+        pq.pop();
+        const int32_t n = e.node;
+        if (placed[n] || e.score != score(n)) continue;  // placed, or a stale score (scores only rise)
+        placed[n] = 1;
+        order.push_back(n);
+        ++t;
+        int32_t o[2];
+        const int m = operands(n, o);
+        for (int i = 0; i < m; ++i) {
+            if (--rem[o[i]] != 1) continue;
+            // the operand's one consumer left now ends its live range
+            for (int32_t j = cstart[o[i]]; j < cstart[o[i] + 1]; ++j) {
+                const int32_t c = cons[j];
+                if (!placed[c] && waiting[c] == 0) pq.push({score(c), ready_at[c], c});
+            }
+        }
+        for (int32_t j = cstart[n]; j < cstart[n + 1]; ++j) {
+            const int32_t c = cons[j];
+            if (--waiting[c] == 0) {
+                ready_at[c] = t;
+                pq.push({score(c), t, c});
+            }
+        }
+    }
+    for (int32_t n = 0; n < N; ++n)
+        if (!placed[n]) order.push_back(n);
+    return order;
 }
 
 PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
@@ -1060,16 +1157,18 @@ PlanSpec build_plan(const SegGraph& g, const PlanOptions& opt, bool emit) {
         if (arity(x.op) >= 1) needed[x.a] = 1;
         if (arity(x.op) >= 2) needed[x.b] = 1;
     }
-    // Partition order.  Without selections it is the recorded order.  With
-    // them, nodes are stably sorted by predicate level (1 + the level of the
+    // Partition order.  Without selections it is the recorded order, or with
+    // option order 1 the list schedule of pressure_order.  With selections,
+    // nodes are stably sorted by predicate level (1 + the level of the
     // popcount a selection reads, the maximum over operands otherwise), so
     // the selections of independent `if`s share kernels instead of each
-    // forcing a cut after its predicate's kernel.  Still a topological order.
+    // forcing a cut after its predicate's kernel.  Always a topological order.
     std::vector<int32_t> order(N), pos(N);
     for (int32_t n = 0; n < N; ++n) order[n] = n;
     {
         bool has_sel = false;
         for (int32_t n = 0; n < N; ++n) has_sel |= needed[n] && g.nodes[n].op == NOp::Sel;
+        if (!has_sel && opt.order == 1) order = pressure_order(g, needed);
         if (has_sel) {
             std::vector<int32_t> level(N, 0);
             for (int32_t n = 0; n < N; ++n) {

@@ -89,6 +89,7 @@ struct PlanOptions {
     bool lazy_loads = false;         // load each input word at its first use in a row, not at the row's start
     bool join_warmup = false;        // a stateful kernel whose step spans its chunk loads warm-up and chunk together
     bool bitop3 = true;              // boolean nodes fused into three-input functions (v_bitop3_b32)
+    int order = 0;                   // 0: partition in recorded order; 1: pressure-aware list schedule
     std::string key() const;
 };
 

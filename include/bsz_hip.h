@@ -150,7 +150,9 @@ int64_t bsz_result_json(const bsz_result* r, char* buf, uint64_t cap);
  * max_ops, max_live, occupancy (0 = automatic), words_per_lane (1 or 2),
  * store_order (1 = default), nontemporal (0, 1 stores, 2 loads and
  * stores), interleave, persistent, lazy_loads, agprs, join_warmup, bitop3
- * (1 = default: three-input boolean folding) and spin_limit.  BSZ_E_RUNTIME
+ * (1 = default: three-input boolean folding), order (0 = default: partition
+ * in recorded order; 1: a list schedule that keeps few values live, so fewer
+ * streams cross kernels) and spin_limit.  BSZ_E_RUNTIME
  * if the program needs host decisions (if/while, int expressions that read a
  * popcount). */
 int bsz_plan_create(bsz_ctx* ctx, const bsz_program* p, const bsz_case* shape, const char* options_json,
